@@ -1,0 +1,331 @@
+// fs_common.h -- device-side building blocks shared by the gfx950 kernels:
+// hash-grid coordinates and hashing (Warp HashGrid semantics), the SPH kernel
+// functions in the reference's float32 evaluation order, and the compacting
+// neighbour loop.
+//
+// Floating point: every file is compiled with -ffp-contract=off and correctly
+// rounded float32 division/sqrt, so each expression below is evaluated exactly
+// as written (no FMA contraction), in the reference's left-to-right order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fluidsim.h"
+
+namespace fs {
+
+constexpr int kOrigin = 1 << 20;
+constexpr float kPi = 3.14159265358979323846f;
+
+// ---------------------------------------------------------------------------
+// error reporting (host)
+// ---------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+int32_t hip_check(hipError_t e, const char* what);
+#define FS_LAUNCH_CHECK(what)                                             \
+    do {                                                                  \
+        hipError_t e_ = hipGetLastError();                                \
+        if (e_ != hipSuccess) return ::fs::hip_check(e_, what);           \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// small float3 algebra with explicit evaluation order
+// ---------------------------------------------------------------------------
+struct v3 {
+    float x, y, z;
+};
+__device__ __forceinline__ v3 V(float x, float y, float z) { return v3{x, y, z}; }
+__device__ __forceinline__ v3 vadd(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 vsub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 smul(float s, v3 a) { return V(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ v3 muls(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 vdivs(v3 a, float s) { return V(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ float vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float vlen(v3 a) { return sqrtf(vdot(a, a)); }
+__device__ __forceinline__ float fmax_(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float fmin_(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fpow_(float a, float b) { return (float)pow((double)a, (double)b); }
+__device__ __forceinline__ v3 ld3(const float* __restrict__ p, int i) { return V(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ __forceinline__ void st3(float* __restrict__ p, int i, v3 v)
+{
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+__device__ __forceinline__ v3 xyz(float4 q) { return V(q.x, q.y, q.z); }
+
+// ParticleFlags.ACTIVE is bit 0 (solver_wcsph.py:53-60).  Sorted copies keep
+// bit 0 = active and bit 1 = dynamic (active and sph:particle_mask == 0).
+constexpr int kActive = 1;
+constexpr int kDynamic = 2;
+
+// ---------------------------------------------------------------------------
+// kernel functions (reference float32 order)
+// ---------------------------------------------------------------------------
+
+// get_cubic  solver_wcsph.py:20-33
+__device__ __forceinline__ float cubic_w(float r_norm, float h)
+{
+    float res = 0.0f;
+    float k = 8.0f / (kPi * (h * h * h));
+    float q = r_norm / h;
+    if (q <= 1.0f) {
+        if (q <= 0.5f) {
+            float q2 = q * q;
+            float q3 = q2 * q;
+            res = k * (6.0f * q3 - 6.0f * q2 + 1.0f);
+        } else {
+            float t = 1.0f - q;
+            res = 2.0f * k * (t * t * t);
+        }
+    }
+    return res;
+}
+
+// get_cubic_derivative  solver_wcsph.py:36-50
+__device__ __forceinline__ v3 cubic_grad(v3 r, float h)
+{
+    v3 res = V(0.0f, 0.0f, 0.0f);
+    float r_norm = vlen(r);
+    float k = 8.0f / (kPi * (h * h * h));
+    float q = r_norm / h;
+    if (r_norm > 1.0e-5f && q <= 1.0f) {
+        v3 grad_q = vdivs(r, r_norm * h);
+        if (q <= 0.5f) {
+            res = smul(6.0f * k * q * (3.0f * q - 2.0f), grad_q);
+        } else {
+            float q_term = 1.0f - q;
+            res = smul(-6.0f * k * q_term * q_term, grad_q);
+        }
+    }
+    return res;
+}
+
+// PBF get_cubic = poly6  solver_pbf.py:23-32
+__device__ __forceinline__ float poly6_w(float r_norm, float h)
+{
+    float res = 0.0f;
+    if (h > 0.0f && r_norm < h) {
+        float h2 = h * h;
+        float rhs = h2 - r_norm * r_norm;
+        float h3 = h * h * h;
+        float k = 315.0f / (64.0f * kPi * (h3 * h3 * h3));
+        res = k * (rhs * rhs * rhs);
+    }
+    return res;
+}
+
+// PBF get_cubic_derivative = spiky gradient  solver_pbf.py:35-45
+__device__ __forceinline__ v3 spiky_grad(v3 r, float h)
+{
+    v3 res = V(0.0f, 0.0f, 0.0f);
+    float r_norm = vlen(r);
+    if (h > 0.0f && r_norm > 1.0e-6f && r_norm < h) {
+        float h3 = h * h * h;
+        float h6 = h3 * h3;
+        float coeff = -45.0f / (kPi * h6);
+        float term = (h - r_norm) * (h - r_norm) / r_norm;
+        res = smul(coeff * term, r);
+    }
+    return res;
+}
+
+// cal_acc_with_non_pressure  solver_wcsph.py:63-89 (= solver_dfsph.py:67-93)
+__device__ __forceinline__ v3 acc_non_pressure(v3 a, float rho_nei, v3 vel, v3 vel_nei, float mass, float mass_nei,
+                                               float mu, float gamma, v3 d, float e_dist, float distance, float h)
+{
+    a = vsub(a, smul(gamma / mass * mass_nei, muls(d, cubic_w(e_dist, h))));
+    float v_current_nei = vdot(vsub(vel, vel_nei), d);
+    float s = 10.0f * mu * (mass_nei / fmax_(rho_nei, 1.0e-6f)) * v_current_nei /
+              (distance * distance + 0.01f * (h * h));
+    return vadd(a, smul(s, cubic_grad(d, h)));
+}
+
+// PBF cal_acc_with_non_pressure  solver_pbf.py:58-87
+__device__ __forceinline__ v3 pbf_acc_non_pressure(v3 a, float rho_nei, v3 vel, v3 vel_nei, float mass,
+                                                   float mass_nei, float mu, float gamma, v3 d, float e_dist,
+                                                   float distance, float h)
+{
+    if (gamma != 0.0f) a = vsub(a, smul(gamma / mass * mass_nei, muls(d, poly6_w(e_dist, h))));
+    if (mu != 0.0f) {
+        float v_current_nei = vdot(vsub(vel, vel_nei), d);
+        if (v_current_nei != 0.0f) {
+            float s = 10.0f * mu * (mass_nei / fmax_(rho_nei, 1.0e-6f)) * v_current_nei /
+                      (distance * distance + 0.01f * (h * h));
+            a = vadd(a, smul(s, spiky_grad(d, h)));
+        }
+    }
+    return a;
+}
+
+// cal_acc_with_pressure  solver_wcsph.py:92-106
+__device__ __forceinline__ v3 acc_pressure(v3 a, v3 d, float p, float p_nei, float rho, float rho_nei,
+                                           float mass_nei, float h)
+{
+    float dp_i = p / fmax_(rho * rho, 1.0e-6f);
+    float dp_nei = p_nei / fmax_(rho_nei * rho_nei, 1.0e-6f);
+    return vadd(a, smul(-mass_nei * (dp_i + dp_nei), cubic_grad(d, h)));
+}
+
+// compute_scorr  solver_pbf.py:90-101
+__device__ __forceinline__ float compute_scorr(float r_norm, float h, float k, float n, float w_delta_q)
+{
+    if (k == 0.0f || w_delta_q <= 1.0e-6f) return 0.0f;
+    float ratio = poly6_w(r_norm, h) / w_delta_q;
+    return -k * fpow_(ratio, n);
+}
+
+// ---------------------------------------------------------------------------
+// hash grid
+// ---------------------------------------------------------------------------
+
+__host__ __device__ __forceinline__ int cell_coord(float v, int round_mode)
+{
+    float f = round_mode ? truncf(v) : floorf(v);
+    if (f > 1.0e9f) f = 1.0e9f;
+    if (f < -1.0e9f) f = -1.0e9f;
+    if (!(f == f)) f = 0.0f;
+    return (int)f;
+}
+
+__host__ __device__ __forceinline__ int hash_axis(int c, int dim)
+{
+    c += kOrigin;
+    if (c < 0) c = 0;
+    return c % dim;
+}
+
+struct GridView {
+    int dx, dy, dz, dxy;
+    int round_mode;
+    float inv;
+    const int* __restrict__ start;  // [dx*dy*dz + 1]
+};
+
+__device__ __forceinline__ int bucket_of(const GridView& g, float x, float y, float z)
+{
+    int cx = cell_coord(x * g.inv, g.round_mode);
+    int cy = cell_coord(y * g.inv, g.round_mode);
+    int cz = cell_coord(z * g.inv, g.round_mode);
+    return hash_axis(cz, g.dz) * g.dxy + hash_axis(cy, g.dy) * g.dx + hash_axis(cx, g.dx);
+}
+
+// Iterator over the sorted-slot ranges visited by hash_grid_query(p, r):
+// rows (y,z) in Warp order; within a row the x run is one contiguous range
+// of sorted slots, or two when the run wraps past the last bucket.
+struct QueryRows {
+    int hx0, seg1, seg2;     // x run: [hx0, hx0+seg1) then [0, seg2)
+    int ys, ny, zs, nz;      // virtual y/z start and extent
+    int yi, zi, part;        // current row
+    int row_base;
+    int k, kend;             // current slot range
+
+    __device__ __forceinline__ void begin(const GridView& g, float px, float py, float pz, float r)
+    {
+        int xs = cell_coord((px - r) * g.inv, g.round_mode);
+        int ysv = cell_coord((py - r) * g.inv, g.round_mode);
+        int zsv = cell_coord((pz - r) * g.inv, g.round_mode);
+        int xe = cell_coord((px + r) * g.inv, g.round_mode);
+        int ye = cell_coord((py + r) * g.inv, g.round_mode);
+        int ze = cell_coord((pz + r) * g.inv, g.round_mode);
+        xe = min(xe, xs + g.dx - 1);
+        ye = min(ye, ysv + g.dy - 1);
+        ze = min(ze, zsv + g.dz - 1);
+        int lenx = xe - xs + 1;
+        ny = ye - ysv + 1;
+        nz = ze - zsv + 1;
+        ys = ysv;
+        zs = zsv;
+        hx0 = hash_axis(xs, g.dx);
+        if (lenx <= 0 || ny <= 0 || nz <= 0) {
+            nz = 0;
+            ny = 1;
+            lenx = 0;
+        }
+        seg1 = min(lenx, g.dx - hx0);
+        seg2 = lenx - seg1;
+        yi = 0;
+        zi = 0;
+        part = 0;
+        k = 0;
+        kend = 0;
+        row_base = 0;
+        if (nz > 0) load_row(g);
+    }
+    __device__ __forceinline__ void load_row(const GridView& g)
+    {
+        row_base = hash_axis(zs + zi, g.dz) * g.dxy + hash_axis(ys + yi, g.dy) * g.dx;
+        int a = row_base + (part == 0 ? hx0 : 0);
+        int b = a + (part == 0 ? seg1 : seg2);
+        k = g.start[a];
+        kend = g.start[b];
+    }
+    // Move to the next row; false when the query is exhausted.
+    __device__ __forceinline__ bool advance(const GridView& g)
+    {
+        if (part == 0 && seg2 > 0) {
+            part = 1;
+        } else {
+            part = 0;
+            if (++yi == ny) {
+                yi = 0;
+                if (++zi == nz) return false;
+            }
+        }
+        load_row(g);
+        return true;
+    }
+    __device__ __forceinline__ bool valid() const { return nz > 0; }
+};
+
+// Visit the candidates of hash_grid_query(p, r) in Warp order, but call
+// `body(k)` (k = sorted slot) only for those with squared distance not above
+// `cut2` -- every skipped candidate has q > 1 and contributes exactly zero to
+// every kernel sum here.  The survivors are compacted through a per-lane LDS
+// list of CAP entries so that the expensive body runs on full waves.
+// `count(k)` is called for every candidate (used for DFSPH's unfiltered
+// neighbour count); pass a no-op when not needed.
+template <int BLOCK, int CAP, class Count, class Body>
+__device__ __forceinline__ void for_each_neighbor(const GridView& g, const float4* __restrict__ pos, float px,
+                                                  float py, float pz, float r, float cut2, int* __restrict__ lds,
+                                                  Count&& count, Body&& body)
+{
+    QueryRows it;
+    it.begin(g, px, py, pz, r);
+    if (!it.valid()) return;
+    int* list = lds + threadIdx.x;
+    bool done = false;
+    while (true) {
+        int cnt = 0;
+        while (cnt < CAP) {
+            if (it.k >= it.kend) {
+                if (!it.advance(g)) {
+                    done = true;
+                    break;
+                }
+                continue;
+            }
+            int k = it.k++;
+            float4 q = pos[k];
+            count(k);
+            float ddx = px - q.x, ddy = py - q.y, ddz = pz - q.z;
+            float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (!(d2 > cut2)) list[(cnt++) * BLOCK] = k;
+        }
+        for (int e = 0; e < cnt; ++e) body(list[e * BLOCK]);
+        if (done) break;
+    }
+}
+
+// Block index -> tile index so that each of the 8 XCDs (which receive
+// workgroups round-robin) processes one contiguous run of tiles.
+__device__ __forceinline__ int xcd_tile()
+{
+    int b = blockIdx.x, nb = gridDim.x;
+    int x = b & 7, q = b >> 3;
+    int per = nb >> 3, rem = nb & 7;
+    return x * per + min(x, rem) + q;
+}
+
+}  // namespace fs

@@ -1,0 +1,118 @@
+// fs_driver.hip -- device reductions of the dam-break driver: CFL maxima and
+// the non-finite state check (examples/dam_break/simulation.py:142-176).
+#include "fs_internal.h"
+
+namespace fs {
+
+constexpr int kRedBlock = 256;
+
+__device__ __forceinline__ float norm2_or_inf(v3 u)
+{
+    float u2 = vdot(u, u);
+    if (!(isfinite(u.x) && isfinite(u.y) && isfinite(u.z) && isfinite(u2))) u2 = INFINITY;
+    return u2;
+}
+
+__global__ void k_fill_zero(uint32_t* p, int n)
+{
+    int i = threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// particle_norm_maxima  simulation.py:142-159.  Values are >= 0 (or +inf), so
+// their float bits order like unsigned integers.
+__global__ __launch_bounds__(kRedBlock) void k_norm_maxima(const float* __restrict__ v, int nv,
+                                                           const float* __restrict__ a, int na, uint32_t* out)
+{
+    __shared__ float s_m[2][kRedBlock / 64];
+    const int n = nv > na ? nv : na;
+    float m0 = 0.0f, m1 = 0.0f;
+    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+        if (i < nv) m0 = fmax_(m0, norm2_or_inf(ld3(v, i)));
+        if (i < na) m1 = fmax_(m1, norm2_or_inf(ld3(a, i)));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        m0 = fmax_(m0, __shfl_xor(m0, off, 64));
+        m1 = fmax_(m1, __shfl_xor(m1, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_m[0][wid] = m0;
+        s_m[1][wid] = m1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kRedBlock / 64; ++w) {
+            m0 = fmax_(m0, s_m[0][w]);
+            m1 = fmax_(m1, s_m[1][w]);
+        }
+        atomicMax(&out[0], __float_as_uint(m0));
+        atomicMax(&out[1], __float_as_uint(m1));
+    }
+}
+
+// check_particle_finite  simulation.py:162-176
+__global__ __launch_bounds__(kRedBlock) void k_check_finite(const float* __restrict__ x, const float* __restrict__ v,
+                                                            int n, const float* __restrict__ rho, int nd,
+                                                            uint32_t* out)
+{
+    int bad = 0;
+    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+        v3 p = ld3(x, i), u = ld3(v, i);
+        bool b = !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(u.x) && isfinite(u.y) &&
+                   isfinite(u.z));
+        if (i < nd) b = b || !isfinite(rho[i]);
+        bad |= b ? 1 : 0;
+    }
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicMax(&out[0], 1u);
+}
+
+static int red_blocks(int n)
+{
+    int b = blocks_for(n, kRedBlock);
+    if (b > 2048) b = 2048;
+    return b > 0 ? b : 1;
+}
+
+}  // namespace fs
+
+using namespace fs;
+
+extern "C" {
+
+int32_t fs_norm_maxima(const float* v, int32_t nv, const float* a, int32_t na, float* maxima, fs_stream_t stream)
+{
+    if (!maxima || nv < 0 || na < 0 || (nv > 0 && !v) || (na > 0 && !a)) {
+        set_error("fs_norm_maxima: invalid arguments");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* out = reinterpret_cast<uint32_t*>(maxima);
+    hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, s, out, 2);
+    FS_LAUNCH_CHECK("k_fill_zero");
+    int n = nv > na ? nv : na;
+    if (n == 0) return FS_OK;
+    hipLaunchKernelGGL(k_norm_maxima, dim3(red_blocks(n)), dim3(kRedBlock), 0, s, v, nv, a, na, out);
+    FS_LAUNCH_CHECK("k_norm_maxima");
+    return FS_OK;
+}
+
+int32_t fs_check_finite(const float* x, const float* v, int32_t n, const float* density, int32_t nd,
+                        int32_t* invalid, fs_stream_t stream)
+{
+    if (!invalid || n < 0 || nd < 0 || nd > n || (n > 0 && (!x || !v)) || (nd > 0 && !density)) {
+        set_error("fs_check_finite: invalid arguments");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* out = reinterpret_cast<uint32_t*>(invalid);
+    hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, s, out, 1);
+    FS_LAUNCH_CHECK("k_fill_zero");
+    if (n == 0) return FS_OK;
+    hipLaunchKernelGGL(k_check_finite, dim3(red_blocks(n)), dim3(kRedBlock), 0, s, x, v, n, density, nd, out);
+    FS_LAUNCH_CHECK("k_check_finite");
+    return FS_OK;
+}
+
+}  // extern "C"

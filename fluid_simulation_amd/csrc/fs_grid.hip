@@ -1,0 +1,509 @@
+// fs_grid.hip -- Warp-compatible uniform hash grid for gfx950.
+//
+// Build = four kernels on one stream, no host sync, no memset:
+//   1. k_hash_count   per original particle: bucket id, rank = atomicAdd(count)
+//   2. k_bucket_scan  single-pass decoupled look-back exclusive scan of the
+//                     dense bucket counts -> cell_start[dims+1]
+//   3. k_scatter      slot = cell_start[b] + rank; re-zero the counter
+//   4. k_fix_gather   order each bucket's members by original index (the
+//                     stable (bucket, index) permutation of Warp's build),
+//                     and gather particle data into that sorted order.
+// The scan's tile counter and descriptors are self-resetting (epoch tagged),
+// so a build can be replayed from a captured graph.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "fs_internal.h"
+
+namespace fs {
+
+static thread_local char g_err[512];
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int32_t hip_check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return FS_OK;
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return FS_ERR_HIP;
+}
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+static inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+GridView grid_view(const fs_grid* g)
+{
+    GridView v;
+    v.dx = g->dim_x;
+    v.dy = g->dim_y;
+    v.dz = g->dim_z;
+    v.dxy = g->dim_x * g->dim_y;
+    v.round_mode = g->round_mode;
+    v.inv = g->cell_width_inv;
+    v.start = g->cell_start;
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+struct HashArgs {
+    GridView g;
+    PreOp pre;
+    int n;
+    int* cell_of;
+    int* rank;
+    int* count;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kElemBlock) void k_hash_count(HashArgs a)
+{
+    const int i = blockIdx.x * kElemBlock + threadIdx.x;
+    if (a.pre.control_reset && i < 16) a.pre.control_reset[i] = 0;
+    if (i >= a.n) return;
+    const PreOp& p = a.pre;
+    v3 x = ld3(p.x_in, i);
+    if (MODE == kPreIntegrate) {
+        // integrate_positions  solver_dfsph.py:340-353
+        bool dyn = (p.flags[i] & 1) != 0 && p.mask[i] == 0;
+        if (dyn) x = vadd(x, muls(ld3(p.v_in, i), p.dt));
+        st3(p.x_out, i, x);
+    } else if (MODE == kPrePredict) {
+        bool dyn = (p.flags[i] & 1) != 0 && p.mask[i] == 0;
+        v3 v = ld3(p.v_in, i);
+        if (dyn) {
+            // predict_velocity_with_gravity  solver_pbf.py:192-208
+            int w = p.world[i];
+            w = w > 0 ? w : 0;
+            v = vadd(v, smul(p.dt, ld3(p.gravity, w)));
+            // predict_positions  solver_pbf.py:211-224
+            x = vadd(x, smul(p.dt, v));
+            // apply_position_bounds  solver_pbf.py:366-399
+            float r = p.particle_radius, e = p.boundary_epsilon;
+            float x_min = -p.bound[0] + r + e, x_max = p.bound[0] - r - e;
+            float y_min = -p.bound[1] + r + e, y_max = p.bound[1] - r - e;
+            float z_min = r + e, z_max = p.bound[2] * 2.0f - r - e;
+            if (x.x < x_min) x.x = x_min;
+            if (x.x > x_max) x.x = x_max;
+            if (x.y < y_min) x.y = y_min;
+            if (x.y > y_max) x.y = y_max;
+            if (x.z < z_min) x.z = z_min;
+            if (x.z > z_max) x.z = z_max;
+        }
+        st3(p.v_out, i, v);
+        st3(p.x_out, i, x);
+    }
+    int c = bucket_of(a.g, x.x, x.y, x.z);
+    a.cell_of[i] = c;
+    a.rank[i] = atomicAdd(&a.count[c], 1);
+}
+
+// Descriptor: [63:62] status (1 aggregate, 2 inclusive prefix),
+//             [61:32] epoch, [31:0] value.
+__device__ __forceinline__ uint64_t scan_desc(uint32_t status, uint32_t epoch, int value)
+{
+    return ((uint64_t)status << 62) | ((uint64_t)(epoch & 0x3FFFFFFFu) << 32) | (uint64_t)(uint32_t)value;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restrict__ in, int* __restrict__ out,
+                                                            int m, int ntiles, uint64_t* desc, uint32_t* sync)
+{
+    __shared__ int s_tile;
+    __shared__ uint32_t s_epoch;
+    __shared__ int s_wsum[kScanBlock / 64];
+    __shared__ int s_excl;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    if (t == 0) {
+        s_tile = (int)atomicAdd(&sync[0], 1u);
+        s_epoch = __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0x3FFFFFFFu;
+    }
+    __syncthreads();
+    const int tile = s_tile;
+    const uint32_t epoch = s_epoch;
+    const int base = tile * kScanTile + t * kScanItems;
+
+    int v[kScanItems];
+    if (base + kScanItems <= m) {
+        const int4* p = reinterpret_cast<const int4*>(in + base);
+#pragma unroll
+        for (int j = 0; j < kScanItems / 4; ++j) {
+            int4 q = p[j];
+            v[4 * j] = q.x;
+            v[4 * j + 1] = q.y;
+            v[4 * j + 2] = q.z;
+            v[4 * j + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) v[j] = (base + j < m) ? in[base + j] : 0;
+    }
+#pragma unroll
+    for (int j = 1; j < kScanItems; ++j) v[j] += v[j - 1];
+    const int tsum = v[kScanItems - 1];
+    int x = tsum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wsum[wid] = x;
+    __syncthreads();
+    int wprefix = 0, agg = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        int ws = s_wsum[w];
+        if (w < wid) wprefix += ws;
+        agg += ws;
+    }
+    const int thread_excl = wprefix + x - tsum;
+
+    if (wid == 0) {
+        int excl = 0;
+        if (tile == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&desc[0], scan_desc(2, epoch, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0)
+                __hip_atomic_store(&desc[tile], scan_desc(1, epoch, agg), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int look = tile - 1;
+            while (true) {
+                int idx = look - lane;
+                int status = 2, val = 0;
+                if (idx >= 0) {
+                    uint64_t d = __hip_atomic_load(&desc[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    uint32_t e = (uint32_t)(d >> 32) & 0x3FFFFFFFu;
+                    status = (e == epoch) ? (int)(d >> 62) : 0;
+                    val = (int)(uint32_t)d;
+                }
+                if (__any(status == 0)) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                unsigned long long pm = __ballot(status == 2);
+                int first = pm ? (__ffsll(pm) - 1) : 64;
+                int contrib = (lane <= first) ? val : 0;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off, 64);
+                excl += contrib;
+                if (pm) break;
+                look -= 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&desc[tile], scan_desc(2, epoch, excl + agg), __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_excl = excl;
+    }
+    __syncthreads();
+    const int excl = s_excl + thread_excl;
+    if (base + kScanItems <= m) {
+        int4* p = reinterpret_cast<int4*>(out + base);
+#pragma unroll
+        for (int j = 0; j < kScanItems / 4; ++j) {
+            int4 q;
+            q.x = excl + (4 * j == 0 ? 0 : v[4 * j - 1]);
+            q.y = excl + v[4 * j];
+            q.z = excl + v[4 * j + 1];
+            q.w = excl + v[4 * j + 2];
+            p[j] = q;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j)
+            if (base + j < m) out[base + j] = excl + (j == 0 ? 0 : v[j - 1]);
+    }
+    if (tile == ntiles - 1 && t == 0) out[m] = s_excl + agg;
+    if (t == 0) {
+        __threadfence();
+        if (atomicAdd(&sync[1], 1u) == (uint32_t)(ntiles - 1)) {
+            __hip_atomic_store(&sync[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&sync[2], (epoch + 1u) & 0x3FFFFFFFu, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __restrict__ cell_of,
+                                                        const int* __restrict__ rank, const int* __restrict__ start,
+                                                        int* __restrict__ scatter_ids, int* __restrict__ count)
+{
+    const int i = blockIdx.x * kElemBlock + threadIdx.x;
+    if (i >= n) return;
+    int c = cell_of[i];
+    scatter_ids[start[c] + rank[i]] = i;
+    count[c] = 0;
+}
+
+__global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int* __restrict__ scatter_ids,
+                                                           const int* __restrict__ cell_of,
+                                                           const int* __restrict__ start, int* __restrict__ point_ids,
+                                                           int* __restrict__ point_cells, GatherOp ga)
+{
+    const int k = blockIdx.x * kElemBlock + threadIdx.x;
+    if (k >= n) return;
+    const int i = scatter_ids[k];
+    const int c = cell_of[i];
+    const int a = start[c], b = start[c + 1];
+    int r = 0;
+    for (int m = a; m < b; ++m) r += (scatter_ids[m] < i) ? 1 : 0;
+    const int f = a + r;
+    point_ids[f] = i;
+    point_cells[f] = c;
+    if (ga.pos) {
+        v3 x = ld3(ga.x, i);
+        float m = ga.mass[i];
+        ga.pos[f] = make_float4(x.x, x.y, x.z, m);
+        float rd = ga.rest_density[i];
+        // compute_particle_volume (solver_wcsph.py:109-116) times rest density,
+        // as compute_density forms mass_nei (solver_wcsph.py:148)
+        ga.wvol[f] = rd * (m / fmax_(rd, 1.0e-6f));
+        int fl = ga.flags[i];
+        int act = fl & 1;
+        ga.sflags[f] = act | ((act && ga.mask[i] == 0) ? kDynamic : 0);
+        if (ga.v) {
+            v3 u = ld3(ga.v, i);
+            ga.vel[f] = make_float4(u.x, u.y, u.z, 0.0f);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kElemBlock) void k_export_tables(int m, const int* __restrict__ start,
+                                                              int* __restrict__ starts, int* __restrict__ ends)
+{
+    const int c = blockIdx.x * kElemBlock + threadIdx.x;
+    if (c >= m) return;
+    int s = start[c], e = start[c + 1];
+    starts[c] = (s == e) ? 0 : s;
+    ends[c] = (s == e) ? 0 : e;
+}
+
+// Diagnostics: candidates of hash_grid_query(points[i], radius), counted or
+// listed in visit order, optionally filtered by the kernels' support test.
+template <bool LIST>
+__global__ __launch_bounds__(kElemBlock) void k_candidates(GridView g, int n, const float* __restrict__ pts,
+                                                           const int* __restrict__ point_ids, float radius, int mode,
+                                                           int* __restrict__ counts,
+                                                           const int64_t* __restrict__ offsets,
+                                                           int* __restrict__ indices)
+{
+    const int i = blockIdx.x * kElemBlock + threadIdx.x;
+    if (i >= n) return;
+    v3 xi = ld3(pts, i);
+    QueryRows it;
+    it.begin(g, xi.x, xi.y, xi.z, radius);
+    int cnt = 0;
+    int64_t o = LIST ? offsets[i] : 0;
+    if (it.valid()) {
+        while (true) {
+            for (int k = it.k; k < it.kend; ++k) {
+                int j = point_ids[k];
+                if (mode >= 0) {
+                    float r = vlen(vsub(xi, ld3(pts, j)));
+                    if (mode == 0 && !(r / radius <= 1.0f)) continue;
+                    if (mode == 1 && !(r < radius)) continue;
+                }
+                if (LIST) indices[o + cnt] = j;
+                ++cnt;
+            }
+            if (!it.advance(g)) break;
+        }
+    }
+    if (!LIST) counts[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+
+static int scan_tiles(const fs_grid* g)
+{
+    long long m = (long long)g->dim_x * g->dim_y * g->dim_z;
+    return (int)((m + kScanTile - 1) / kScanTile);
+}
+
+int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const GatherOp& ga, hipStream_t s)
+{
+    if (n < 0 || n > g->capacity) {
+        set_error("grid build: %d points exceed the reserved capacity %d", n, g->capacity);
+        return FS_ERR_CAPACITY;
+    }
+    if (!(radius > 0.0f)) {
+        set_error("grid build: radius must be positive");
+        return FS_ERR_ARG;
+    }
+    g->num_points = n;
+    g->cell_width = radius;
+    g->cell_width_inv = 1.0f / radius;
+    const GridView gv = grid_view(g);
+    const int m = g->dim_x * g->dim_y * g->dim_z;
+    HashArgs ha{gv, pre, n, g->cell_of, g->rank, g->cell_count};
+    const int nb = blocks_for(n, kElemBlock);
+    if (nb > 0 || pre.control_reset) {
+        const int nbh = nb > 0 ? nb : 1;
+        switch (pre.mode) {
+        case kPreIntegrate: hipLaunchKernelGGL(k_hash_count<kPreIntegrate>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
+        case kPrePredict: hipLaunchKernelGGL(k_hash_count<kPrePredict>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
+        default: hipLaunchKernelGGL(k_hash_count<kPrePlain>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
+        }
+        FS_LAUNCH_CHECK("k_hash_count");
+    }
+    const int ntiles = scan_tiles(g);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(ntiles), dim3(kScanBlock), 0, s, g->cell_count, g->cell_start, m, ntiles,
+                       g->scan_state, g->sync);
+    FS_LAUNCH_CHECK("k_bucket_scan");
+    if (nb == 0) return FS_OK;
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kElemBlock), 0, s, n, g->cell_of, g->rank, g->cell_start,
+                       g->scatter_ids, g->cell_count);
+    FS_LAUNCH_CHECK("k_scatter");
+    hipLaunchKernelGGL(k_fix_gather, dim3(nb), dim3(kElemBlock), 0, s, n, g->scatter_ids, g->cell_of, g->cell_start,
+                       g->point_ids, g->point_cells, ga);
+    FS_LAUNCH_CHECK("k_fix_gather");
+    return FS_OK;
+}
+
+}  // namespace fs
+
+using namespace fs;
+
+extern "C" {
+
+const char* fs_last_error(void) { return g_err; }
+
+int32_t fs_version(void) { return 100; }
+
+size_t fs_grid_workspace_bytes(int32_t capacity, int32_t dim_x, int32_t dim_y, int32_t dim_z)
+{
+    if (capacity < 0 || dim_x <= 0 || dim_y <= 0 || dim_z <= 0) return 0;
+    size_t m = (size_t)dim_x * dim_y * dim_z;
+    size_t cap = (size_t)(capacity > 0 ? capacity : 1);
+    size_t ntiles = (m + kScanTile - 1) / kScanTile;
+    size_t b = 0;
+    b += align_up(cap * 4) * 5;                 // point_ids, point_cells, cell_of, rank, scatter_ids
+    b += align_up((m + 1) * 4);                 // cell_start
+    b += align_up(m * 4);                       // cell_count
+    b += align_up(ntiles * 8);                  // scan_state
+    b += align_up(8 * 4);                       // sync
+    return b;
+}
+
+int32_t fs_grid_init(fs_grid* g, void* workspace, size_t bytes, int32_t capacity, int32_t dim_x, int32_t dim_y,
+                     int32_t dim_z, int32_t round_mode, fs_stream_t stream)
+{
+    if (!g || !workspace || capacity < 0 || dim_x <= 0 || dim_y <= 0 || dim_z <= 0) {
+        set_error("fs_grid_init: invalid arguments");
+        return FS_ERR_ARG;
+    }
+    if ((long long)dim_x * dim_y * dim_z >= (1LL << 30)) {
+        set_error("fs_grid_init: grid dims too large");
+        return FS_ERR_ARG;
+    }
+    size_t need = fs_grid_workspace_bytes(capacity, dim_x, dim_y, dim_z);
+    if (bytes < need) {
+        set_error("fs_grid_init: workspace %zu bytes < required %zu", bytes, need);
+        return FS_ERR_WORKSPACE;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    size_t m = (size_t)dim_x * dim_y * dim_z;
+    size_t cap = (size_t)(capacity > 0 ? capacity : 1);
+    size_t ntiles = (m + kScanTile - 1) / kScanTile;
+    char* p = (char*)workspace;
+    auto take = [&](size_t nbytes) {
+        char* r = p;
+        p += align_up(nbytes);
+        return r;
+    };
+    memset(g, 0, sizeof(*g));
+    g->dim_x = dim_x;
+    g->dim_y = dim_y;
+    g->dim_z = dim_z;
+    g->round_mode = round_mode ? 1 : 0;
+    g->capacity = capacity;
+    g->point_ids = (int32_t*)take(cap * 4);
+    g->point_cells = (int32_t*)take(cap * 4);
+    g->cell_of = (int32_t*)take(cap * 4);
+    g->rank = (int32_t*)take(cap * 4);
+    g->scatter_ids = (int32_t*)take(cap * 4);
+    g->cell_start = (int32_t*)take((m + 1) * 4);
+    g->cell_count = (int32_t*)take(m * 4);
+    g->scan_state = (uint64_t*)take(ntiles * 8);
+    g->sync = (uint32_t*)take(8 * 4);
+    int32_t rc = hip_check(hipMemsetAsync(g->cell_count, 0, m * 4, s), "fs_grid_init memset");
+    if (rc) return rc;
+    rc = hip_check(hipMemsetAsync(g->cell_start, 0, (m + 1) * 4, s), "fs_grid_init memset");
+    if (rc) return rc;
+    rc = hip_check(hipMemsetAsync(g->scan_state, 0, ntiles * 8, s), "fs_grid_init memset");
+    if (rc) return rc;
+    return hip_check(hipMemsetAsync(g->sync, 0, 8 * 4, s), "fs_grid_init memset");
+}
+
+int32_t fs_grid_build(fs_grid* g, const float* points, int32_t n, float radius, fs_stream_t stream)
+{
+    if (!g || (n > 0 && !points)) {
+        set_error("fs_grid_build: invalid arguments");
+        return FS_ERR_ARG;
+    }
+    PreOp pre;
+    pre.mode = kPrePlain;
+    pre.x_in = points;
+    GatherOp ga;
+    return build_grid(g, n, radius, pre, ga, (hipStream_t)stream);
+}
+
+int32_t fs_grid_export_tables(const fs_grid* g, int32_t* cell_starts, int32_t* cell_ends, fs_stream_t stream)
+{
+    if (!g || !cell_starts || !cell_ends) {
+        set_error("fs_grid_export_tables: invalid arguments");
+        return FS_ERR_ARG;
+    }
+    int m = g->dim_x * g->dim_y * g->dim_z;
+    hipLaunchKernelGGL(k_export_tables, dim3(blocks_for(m, kElemBlock)), dim3(kElemBlock), 0, (hipStream_t)stream, m,
+                       g->cell_start, cell_starts, cell_ends);
+    FS_LAUNCH_CHECK("k_export_tables");
+    return FS_OK;
+}
+
+int32_t fs_grid_count_candidates(const fs_grid* g, const float* points, float radius, int32_t within_mode,
+                                 int32_t* counts, fs_stream_t stream)
+{
+    if (!g || !counts || (g->num_points > 0 && !points) || within_mode < -1 || within_mode > 1) {
+        set_error("fs_grid_count_candidates: invalid arguments");
+        return FS_ERR_ARG;
+    }
+    int n = g->num_points;
+    if (n == 0) return FS_OK;
+    hipLaunchKernelGGL(k_candidates<false>, dim3(blocks_for(n, kElemBlock)), dim3(kElemBlock), 0,
+                       (hipStream_t)stream, grid_view(g), n, points, g->point_ids, radius, within_mode, counts,
+                       (const int64_t*)nullptr, (int*)nullptr);
+    FS_LAUNCH_CHECK("k_candidates<count>");
+    return FS_OK;
+}
+
+int32_t fs_grid_list_candidates(const fs_grid* g, const float* points, float radius, int32_t within_mode,
+                                const int64_t* offsets, int32_t* indices, fs_stream_t stream)
+{
+    if (!g || !offsets || !indices || (g->num_points > 0 && !points) || within_mode < -1 || within_mode > 1) {
+        set_error("fs_grid_list_candidates: invalid arguments");
+        return FS_ERR_ARG;
+    }
+    int n = g->num_points;
+    if (n == 0) return FS_OK;
+    hipLaunchKernelGGL(k_candidates<true>, dim3(blocks_for(n, kElemBlock)), dim3(kElemBlock), 0, (hipStream_t)stream,
+                       grid_view(g), n, points, g->point_ids, radius, within_mode, (int*)nullptr, offsets, indices);
+    FS_LAUNCH_CHECK("k_candidates<list>");
+    return FS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1253 @@
+// fs_sph.hip -- neighbour-loop passes and substep orchestration of the WCSPH,
+// DFSPH and PBF solvers on gfx950.
+//
+// Each pass is one launch of one lane per particle, lanes in the grid's sorted
+// order (slot s -> original index i = point_ids[s]) and workgroups mapped so
+// each XCD walks one contiguous run of slots.  Neighbour data is read from
+// copies gathered into sorted order (contiguous bucket rows); per-particle
+// results are written at the original index, and neighbour-visible results
+// also into sorted scratch for the next pass.  Sums run over candidates in
+// hash_grid_query visit order with the reference's float32 expressions, so a
+// pass reproduces the reference kernel's arithmetic operation for operation.
+#include "fs_internal.h"
+
+namespace fs {
+
+constexpr int kPassBlock = 256;
+constexpr int kListCap = 32;
+constexpr float kCutSlack = 1.0001f;   // candidates with d2 > h^2*slack have q > 1
+
+// loop-control words in fs_work::control
+enum Control : int {
+    kStopRho = 0,
+    kStopVel = 1,
+    kItRho = 2,
+    kCorrRho = 3,
+    kItVel = 4,
+    kCorrVel = 5,
+    kTicket = 8,
+};
+
+struct PassCommon {
+    GridView g;
+    int n;
+    const int* __restrict__ perm;
+    const float4* __restrict__ pos;
+    const int* __restrict__ sflags;
+    float h;
+    float cut2;
+};
+
+__device__ __forceinline__ int pass_slot() { return xcd_tile() * kPassBlock + threadIdx.x; }
+
+struct NoCount {
+    __device__ __forceinline__ void operator()(int) const {}
+};
+
+// ---------------------------------------------------------------------------
+// block reduction of the DFSPH residual; the last block decides the early exit
+// ---------------------------------------------------------------------------
+__device__ void residual_reduce(double v, double c, double* partials, int* control, int stop_idx, int it_idx,
+                                int iter, double eta)
+{
+    __shared__ double s_v[kPassBlock / 64];
+    __shared__ double s_c[kPassBlock / 64];
+    __shared__ int s_last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, 64);
+        c += __shfl_xor(c, off, 64);
+    }
+    if (lane == 0) {
+        s_v[wid] = v;
+        s_c[wid] = c;
+    }
+    __syncthreads();
+    const int nb = gridDim.x;
+    if (threadIdx.x == 0) {
+        double bv = 0.0, bc = 0.0;
+        for (int w = 0; w < kPassBlock / 64; ++w) {
+            bv += s_v[w];
+            bc += s_c[w];
+        }
+        __hip_atomic_store(&partials[2 * blockIdx.x], bv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[2 * blockIdx.x + 1], bc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        int t = atomicAdd(&control[kTicket], 1);
+        s_last = (t == nb - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    double tv = 0.0, tc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kPassBlock) {
+        tv += __hip_atomic_load(&partials[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tc += __hip_atomic_load(&partials[2 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        tv += __shfl_xor(tv, off, 64);
+        tc += __shfl_xor(tc, off, 64);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        s_v[wid] = tv;
+        s_c[wid] = tc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sv = 0.0, sc = 0.0;
+        for (int w = 0; w < kPassBlock / 64; ++w) {
+            sv += s_v[w];
+            sc += s_c[w];
+        }
+        double residual = sc > 0.0 ? sv / sc : 0.0;
+        control[it_idx] += 1;
+        if (iter > 0 && residual <= eta) control[stop_idx] = 1;
+        control[kTicket] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// WCSPH
+// ---------------------------------------------------------------------------
+
+struct WcsphDensityArgs {
+    PassCommon c;
+    const float* __restrict__ wvol;
+    const float* __restrict__ rest_density;
+    const float* __restrict__ stiffness;
+    const float* __restrict__ exponent;
+    int clamp;
+    float* rho;
+    float* pressure;
+    float4* sca;   // rho, p
+};
+
+// compute_density_pressure  solver_wcsph.py:119-158
+__global__ __launch_bounds__(kPassBlock) void k_wcsph_density(WcsphDensityArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    if (!(a.c.sflags[s] & kActive)) {
+        a.rho[i] = 0.0f;
+        a.pressure[i] = 0.0f;
+        a.sca[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float4 xi = a.c.pos[s];
+    const float h = a.c.h;
+    float rho_temp = 0.0f;
+    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
+                                            [&](int k) {
+                                                if (!(a.c.sflags[k] & kActive)) return;
+                                                v3 dist = vsub(xyz(xi), xyz(a.c.pos[k]));
+                                                float mass_nei = a.wvol[k];
+                                                rho_temp += mass_nei * cubic_w(vlen(dist), h);
+                                            });
+    const float rho_i = rho_temp;
+    float p = a.stiffness[i] * (fpow_(rho_i / fmax_(a.rest_density[i], 1.0e-6f), a.exponent[i]) - 1.0f);
+    if (a.clamp) p = fmax_(p, 0.0f);
+    a.rho[i] = rho_i;
+    a.pressure[i] = p;
+    a.sca[s] = make_float4(rho_i, p, 0.0f, 0.0f);
+}
+
+struct WcsphForceArgs {
+    PassCommon c;
+    const float4* __restrict__ vel;
+    const float4* __restrict__ sca;
+    const float* __restrict__ surface_tension;
+    const float* __restrict__ viscosity;
+    const float* __restrict__ gravity;
+    const int* __restrict__ world;
+    const float* __restrict__ x_in;
+    const float* __restrict__ v_in;
+    float particle_radius;
+    float dt;
+    float bound[3];
+    float damping;
+    float* accel;
+    float* x_out;
+    float* v_out;
+};
+
+// acceleration_wcsph + integrate_velocity + integrate_positions + apply_bounds
+// solver_wcsph.py:161-305
+__global__ __launch_bounds__(kPassBlock) void k_wcsph_force(WcsphForceArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    if (!(a.c.sflags[s] & kDynamic)) {
+        st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
+        st3(a.v_out, i, ld3(a.v_in, i));
+        st3(a.x_out, i, ld3(a.x_in, i));
+        return;
+    }
+    const float4 xi4 = a.c.pos[s];
+    const v3 xi = xyz(xi4);
+    const v3 vi = xyz(a.vel[s]);
+    const float4 si = a.sca[s];
+    const float rho_i = si.x, p_i = si.y, m_i = xi4.w;
+    const float mu = a.viscosity[i], gamma = a.surface_tension[i];
+    const float h = a.c.h, pr = a.particle_radius;
+    v3 acc = V(0.0f, 0.0f, 0.0f);
+    for_each_neighbor<kPassBlock, kListCap>(
+        a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{}, [&](int k) {
+            if (k == s || !(a.c.sflags[k] & kActive)) return;
+            const float4 xj = a.c.pos[k];
+            const float4 sj = a.sca[k];
+            v3 direction = vsub(xi, xyz(xj));
+            float distance = vlen(direction);
+            float mass_nei = xj.w;
+            float rho_nei = sj.x;
+            float p_nei = sj.y;
+            acc = acc_non_pressure(acc, rho_nei, vi, xyz(a.vel[k]), m_i, mass_nei, mu, gamma, direction,
+                                   fmax_(distance, pr), distance, h);
+            acc = acc_pressure(acc, direction, p_i, p_nei, rho_i, rho_nei, mass_nei, h);
+        });
+    int w = a.world[i];
+    w = w > 0 ? w : 0;
+    const v3 acc_g = vadd(acc, ld3(a.gravity, w));
+    st3(a.accel, i, acc_g);
+    v3 v = vadd(vi, smul(a.dt, acc_g));
+    v3 x = vadd(xi, muls(v, a.dt));
+    const float sx = a.bound[0], sy = a.bound[1], sz = a.bound[2], d = a.damping;
+    if (x.x < -sx) { x.x = -sx; if (v.x < 0.0f) v.x = v.x * d; }
+    if (x.x > sx) { x.x = sx; if (v.x > 0.0f) v.x = v.x * d; }
+    if (x.y < -sy) { x.y = -sy; if (v.y < 0.0f) v.y = v.y * d; }
+    if (x.y > sy) { x.y = sy; if (v.y > 0.0f) v.y = v.y * d; }
+    if (x.z < 0.0f) { x.z = 0.0f; if (v.z < 0.0f) v.z = v.z * d; }
+    if (x.z > sz * 2.0f) { x.z = sz * 2.0f; if (v.z > 0.0f) v.z = v.z * d; }
+    st3(a.v_out, i, v);
+    st3(a.x_out, i, x);
+}
+
+// ---------------------------------------------------------------------------
+// DFSPH
+// ---------------------------------------------------------------------------
+
+struct DfDensityArgs {
+    PassCommon c;
+    const float* __restrict__ wvol;
+    int min_neighbors;
+    const float* __restrict__ kappa_warm;   // warm start source (state_in.sph.kappa / kappa_v) or null
+    float* kappa_commit;                    // state_out.sph.kappa / kappa_v when warm starting
+    float* rho;
+    float* factor;
+    int* neighbor_count;
+    float4* sca;   // rho, factor, kappa, kappa/max(rho,1e-6)
+};
+
+// compute_density + compute_factor_dfsph  solver_dfsph.py:106-174 (one pass)
+__global__ __launch_bounds__(kPassBlock) void k_dfsph_density_factor(DfDensityArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    if (!(a.c.sflags[s] & kActive)) {
+        a.rho[i] = 0.0f;
+        a.factor[i] = 0.0f;
+        a.neighbor_count[i] = 0;
+        float kw = a.kappa_warm ? a.kappa_warm[i] : 0.0f;
+        if (a.kappa_warm) a.kappa_commit[i] = kw;
+        a.sca[s] = make_float4(0.0f, 0.0f, kw, kw / fmax_(0.0f, 1.0e-6f));
+        return;
+    }
+    const float4 xi4 = a.c.pos[s];
+    const v3 xi = xyz(xi4);
+    const float h = a.c.h;
+    float rho_temp = 0.0f;
+    v3 sum_grad = V(0.0f, 0.0f, 0.0f);
+    float sum_grad_sq = 0.0f;
+    int count = 0;
+    auto counter = [&](int k) { count += (k != s && (a.c.sflags[k] & kActive)) ? 1 : 0; };
+    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, counter,
+                                            [&](int k) {
+                                                if (!(a.c.sflags[k] & kActive)) return;
+                                                const float4 xj = a.c.pos[k];
+                                                v3 r = vsub(xi, xyz(xj));
+                                                rho_temp += a.wvol[k] * cubic_w(vlen(r), h);
+                                                v3 grad = smul(xj.w, cubic_grad(r, h));
+                                                sum_grad = vadd(sum_grad, grad);
+                                                sum_grad_sq += vdot(grad, grad);
+                                            });
+    const float denom = vdot(sum_grad, sum_grad) + sum_grad_sq;
+    const float f = (count < a.min_neighbors || denom < 1.0e-6f) ? 0.0f : 1.0f / denom;
+    a.rho[i] = rho_temp;
+    a.factor[i] = f;
+    a.neighbor_count[i] = count;
+    float kw = 0.0f;
+    if (a.kappa_warm) {
+        kw = a.kappa_warm[i];
+        a.kappa_commit[i] = kw;
+    }
+    a.sca[s] = make_float4(rho_temp, f, kw, kw / fmax_(rho_temp, 1.0e-6f));
+}
+
+struct DfNonPressureArgs {
+    PassCommon c;
+    const float4* __restrict__ vel;
+    const float4* __restrict__ sca;
+    const float* __restrict__ surface_tension;
+    const float* __restrict__ viscosity;
+    const float* __restrict__ gravity;
+    const int* __restrict__ world;
+    float particle_radius;
+    float dt;
+    float* accel;
+    float* v_star;
+    float4* vstar_s;
+};
+
+// acceleration_non_pressure + predict_velocity  solver_dfsph.py:177-242
+__global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    if (!(a.c.sflags[s] & kDynamic)) {
+        st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
+        st3(a.v_star, i, V(0.0f, 0.0f, 0.0f));
+        a.vstar_s[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float4 xi4 = a.c.pos[s];
+    const v3 xi = xyz(xi4);
+    const v3 vi = xyz(a.vel[s]);
+    const float m_i = xi4.w, mu = a.viscosity[i], gamma = a.surface_tension[i];
+    const float h = a.c.h, pr = a.particle_radius;
+    v3 acc = V(0.0f, 0.0f, 0.0f);
+    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
+                                            [&](int k) {
+                                                if (k == s || !(a.c.sflags[k] & kActive)) return;
+                                                const float4 xj = a.c.pos[k];
+                                                v3 d = vsub(xi, xyz(xj));
+                                                float dist = vlen(d);
+                                                acc = acc_non_pressure(acc, a.sca[k].x, vi, xyz(a.vel[k]), m_i, xj.w,
+                                                                       mu, gamma, d, fmax_(dist, pr), dist, h);
+                                            });
+    int w = a.world[i];
+    w = w > 0 ? w : 0;
+    const v3 acc_g = vadd(acc, ld3(a.gravity, w));
+    st3(a.accel, i, acc_g);
+    const v3 vs = vadd(vi, smul(a.dt, acc_g));
+    st3(a.v_star, i, vs);
+    a.vstar_s[s] = make_float4(vs.x, vs.y, vs.z, 0.0f);
+}
+
+struct DfRateArgs {
+    PassCommon c;
+    const float4* __restrict__ vstar_s;
+    const float* __restrict__ rest_density;
+    float dt;
+    float max_kappa;
+    int density_loop;   // 1: density loop (rho_star, kappa), 0: divergence loop (kappa_v)
+    int iter;
+    double eta;
+    float* d_rho_dt;
+    float* rho_star;
+    float4* sca;
+    double* partials;
+    int* control;
+};
+
+// compute_density_change_rate (+ predict_density_star, compute_kappa_density
+// or compute_kappa_divergence, and the host residual)  solver_dfsph.py:245-306,
+// 356-372, 579-596
+__global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int stop_idx = a.density_loop ? kStopRho : kStopVel;
+    if (a.control[stop_idx]) return;
+    const int s = pass_slot();
+    double res_v = 0.0, res_c = 0.0;
+    if (s < a.c.n) {
+        const int i = a.c.perm[s];
+        const int fl = a.c.sflags[s];
+        float4 si = a.sca[s];
+        if (!(fl & kActive)) {
+            a.d_rho_dt[i] = 0.0f;
+            if (a.density_loop) a.rho_star[i] = 0.0f;
+            si.z = 0.0f;
+            si.w = 0.0f / fmax_(si.x, 1.0e-6f);
+            a.sca[s] = si;
+        } else {
+            const v3 xi = xyz(a.c.pos[s]);
+            const v3 vi = xyz(a.vstar_s[s]);
+            const float h = a.c.h;
+            float div_val = 0.0f;
+            for_each_neighbor<kPassBlock, kListCap>(
+                a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{}, [&](int k) {
+                    if (!(a.c.sflags[k] & kActive)) return;
+                    const float4 xj = a.c.pos[k];
+                    v3 grad_w = cubic_grad(vsub(xi, xyz(xj)), h);
+                    div_val += xj.w * vdot(vsub(vi, xyz(a.vstar_s[k])), grad_w);
+                });
+            a.d_rho_dt[i] = div_val;
+            const bool dyn = (fl & kDynamic) != 0;
+            float kap = 0.0f;
+            if (a.density_loop) {
+                const float rs = si.x + a.dt * div_val;
+                a.rho_star[i] = rs;
+                const float err = fmax_(rs - a.rest_density[i], 0.0f);
+                if (dyn) {
+                    res_v = (double)err;
+                    res_c = 1.0;
+                    kap = fmin_((err / (a.dt * a.dt)) * si.y, a.max_kappa);
+                }
+            } else {
+                const float dc = fmax_(div_val, 0.0f);
+                if (dyn) {
+                    res_v = (double)dc;
+                    res_c = 1.0;
+                    kap = fmin_((1.0f / a.dt) * dc * si.x * si.y, a.max_kappa);
+                }
+            }
+            si.z = kap;
+            si.w = kap / fmax_(si.x, 1.0e-6f);
+            a.sca[s] = si;
+        }
+    }
+    residual_reduce(res_v, res_c, a.partials, a.control, stop_idx, a.density_loop ? kItRho : kItVel, a.iter, a.eta);
+}
+
+struct DfCorrArgs {
+    PassCommon c;
+    const float4* __restrict__ sca;
+    float dt;
+    int stop_idx;   // -1: unconditional (warm start)
+    int corr_idx;
+    float* v_star;
+    float4* vstar_s;
+    float* kappa_commit;
+    int* control;
+};
+
+// apply_constant_density_correction / apply_divergence_free_correction
+// solver_dfsph.py:309-337, 375-403 (and the kappa write of :290-306, 356-372)
+__global__ __launch_bounds__(kPassBlock) void k_dfsph_correction(DfCorrArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    if (a.stop_idx >= 0 && a.control[a.stop_idx]) return;
+    const int s = pass_slot();
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.corr_idx >= 0) a.control[a.corr_idx] += 1;
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    const float4 si = a.sca[s];
+    if (a.kappa_commit) a.kappa_commit[i] = si.z;
+    if (!(a.c.sflags[s] & kDynamic)) return;
+    const v3 xi = xyz(a.c.pos[s]);
+    const float h = a.c.h;
+    const float kri = si.w;
+    v3 dv = V(0.0f, 0.0f, 0.0f);
+    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
+                                            [&](int k) {
+                                                if (!(a.c.sflags[k] & kActive)) return;
+                                                const float4 xj = a.c.pos[k];
+                                                float coeff = xj.w * (kri + a.sca[k].w);
+                                                dv = vadd(dv, smul(coeff, cubic_grad(vsub(xi, xyz(xj)), h)));
+                                            });
+    const float4 vs4 = a.vstar_s[s];
+    const v3 vs = vsub(xyz(vs4), smul(a.dt, dv));
+    st3(a.v_star, i, vs);
+    a.vstar_s[s] = make_float4(vs.x, vs.y, vs.z, 0.0f);
+}
+
+struct DfFinalArgs {
+    int n;
+    const int* __restrict__ flags;
+    const int* __restrict__ mask;
+    const float* __restrict__ v_star;
+    float* x_out;
+    float* v_out;
+    float bound[3];
+    float damping;
+    const int* control;
+    int* iterations;
+};
+
+// copy v_star -> particle_qd, apply_bounds (DFSPH variant)  solver_dfsph.py:406-442,885-898
+__global__ __launch_bounds__(kElemBlock) void k_dfsph_finalize(DfFinalArgs a)
+{
+    const int t = blockIdx.x * kElemBlock + threadIdx.x;
+    if (a.iterations && t < 4) a.iterations[t] = a.control[kItRho + t];
+    if (t >= a.n) return;
+    v3 v = ld3(a.v_star, t);
+    if (!((a.flags[t] & 1) != 0 && a.mask[t] == 0)) {
+        st3(a.v_out, t, v);
+        return;
+    }
+    v3 x = ld3(a.x_out, t);
+    const float sx = a.bound[0], sy = a.bound[1], sz = a.bound[2], d = a.damping;
+    if (x.x < -sx) { x.x = -sx; v.x = v.x * d; }
+    if (x.x > sx) { x.x = sx; v.x = v.x * d; }
+    if (x.y < -sy) { x.y = -sy; v.y = v.y * d; }
+    if (x.y > sy) { x.y = sy; v.y = v.y * d; }
+    if (x.z < 0.0f) { x.z = 0.0f; v.z = v.z * d; }
+    if (x.z > sz * 2.0f) { x.z = sz * 2.0f; v.z = v.z * d; }
+    st3(a.x_out, t, x);
+    st3(a.v_out, t, v);
+}
+
+// ---------------------------------------------------------------------------
+// PBF
+// ---------------------------------------------------------------------------
+
+struct PbfBounds {
+    float bound[3];
+    float particle_radius;
+    float boundary_epsilon;
+    __device__ __forceinline__ v3 clamp(v3 x) const
+    {
+        // apply_position_bounds  solver_pbf.py:366-399
+        float r = particle_radius, e = boundary_epsilon;
+        float x_min = -bound[0] + r + e, x_max = bound[0] - r - e;
+        float y_min = -bound[1] + r + e, y_max = bound[1] - r - e;
+        float z_min = r + e, z_max = bound[2] * 2.0f - r - e;
+        if (x.x < x_min) x.x = x_min;
+        if (x.x > x_max) x.x = x_max;
+        if (x.y < y_min) x.y = y_min;
+        if (x.y > y_max) x.y = y_max;
+        if (x.z < z_min) x.z = z_min;
+        if (x.z > z_max) x.z = z_max;
+        return x;
+    }
+    // apply_bounds  solver_pbf.py:466-517
+    __device__ __forceinline__ void clamp_damp(v3& x, v3& v, float d) const
+    {
+        float r = particle_radius, e = boundary_epsilon;
+        float x_min = -bound[0] + r + e, x_max = bound[0] - r - e;
+        float y_min = -bound[1] + r + e, y_max = bound[1] - r - e;
+        float z_min = r + e, z_max = bound[2] * 2.0f - r - e;
+        if (x.x < x_min) { x.x = x_min; if (v.x < 0.0f) v.x = v.x * d; }
+        if (x.x > x_max) { x.x = x_max; if (v.x > 0.0f) v.x = v.x * d; }
+        if (x.y < y_min) { x.y = y_min; if (v.y < 0.0f) v.y = v.y * d; }
+        if (x.y > y_max) { x.y = y_max; if (v.y > 0.0f) v.y = v.y * d; }
+        if (x.z < z_min) { x.z = z_min; if (v.z < 0.0f) v.z = v.z * d; }
+        if (x.z > z_max) { x.z = z_max; if (v.z > 0.0f) v.z = v.z * d; }
+    }
+};
+
+struct PbfDensityArgs {
+    PassCommon c;
+    const float* __restrict__ wvol;
+    float* rho;
+    float4* sca;
+    // final-density mode: update_velocity_from_positions (+ finalize when no XSPH)
+    int final_pass;
+    int finalize;
+    const float* __restrict__ x_in;
+    const float* __restrict__ v_in;
+    float dt;
+    float damping;
+    PbfBounds bounds;
+    float4* vel;
+    float* x_out;
+    float* v_out;
+};
+
+// compute_density (poly6)  solver_pbf.py:114-141; in the final pass also
+// update_velocity_from_positions :402-416 and, without XSPH, :939-954
+__global__ __launch_bounds__(kPassBlock) void k_pbf_density(PbfDensityArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    const int fl = a.c.sflags[s];
+    const float4 xi4 = a.c.pos[s];
+    const v3 xi = xyz(xi4);
+    float rho_temp = 0.0f;
+    if (fl & kActive) {
+        const float h = a.c.h;
+        for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
+                                                [&](int k) {
+                                                    if (!(a.c.sflags[k] & kActive)) return;
+                                                    v3 r = vsub(xi, xyz(a.c.pos[k]));
+                                                    rho_temp += a.wvol[k] * poly6_w(vlen(r), h);
+                                                });
+    }
+    a.rho[i] = rho_temp;
+    a.sca[s] = make_float4(rho_temp, 0.0f, 0.0f, 0.0f);
+    if (!a.final_pass) return;
+    v3 v;
+    const bool dyn = (fl & kDynamic) != 0;
+    if (dyn) v = vdivs(vsub(xi, ld3(a.x_in, i)), fmax_(a.dt, 1.0e-8f));
+    else v = ld3(a.v_in, i);
+    a.vel[s] = make_float4(v.x, v.y, v.z, 0.0f);
+    if (a.finalize) {
+        v3 x = xi;
+        if (dyn) a.bounds.clamp_damp(x, v, a.damping);
+        st3(a.x_out, i, x);
+        st3(a.v_out, i, v);
+    }
+}
+
+struct PbfNonPressureArgs {
+    PassCommon c;
+    const float4* __restrict__ vel;
+    const float4* __restrict__ sca;
+    const float* __restrict__ surface_tension;
+    const float* __restrict__ viscosity;
+    float particle_radius;
+    float dt;
+    PbfBounds bounds;
+    float* accel;
+    float* x_pred;
+    float* v_pred;
+    float4* pos_next;
+};
+
+// acceleration_non_pressure (PBF) + apply_non_pressure_prediction +
+// apply_position_bounds  solver_pbf.py:144-189, 227-242, 784-796
+__global__ __launch_bounds__(kPassBlock) void k_pbf_nonpressure(PbfNonPressureArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    const float4 xi4 = a.c.pos[s];
+    if (!(a.c.sflags[s] & kDynamic)) {
+        st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
+        a.pos_next[s] = xi4;
+        return;
+    }
+    const v3 xi = xyz(xi4);
+    const v3 vi = xyz(a.vel[s]);
+    const float m_i = xi4.w, mu = a.viscosity[i], gamma = a.surface_tension[i];
+    const float h = a.c.h, pr = a.particle_radius;
+    v3 acc = V(0.0f, 0.0f, 0.0f);
+    if (gamma != 0.0f || mu != 0.0f) {
+        for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
+                                                [&](int k) {
+                                                    if (k == s || !(a.c.sflags[k] & kActive)) return;
+                                                    const float4 xj = a.c.pos[k];
+                                                    v3 d = vsub(xi, xyz(xj));
+                                                    float dist = vlen(d);
+                                                    acc = pbf_acc_non_pressure(acc, a.sca[k].x, vi, xyz(a.vel[k]),
+                                                                               m_i, xj.w, mu, gamma, d,
+                                                                               fmax_(dist, pr), dist, h);
+                                                });
+    }
+    st3(a.accel, i, acc);
+    const v3 delta_v = smul(a.dt, acc);
+    const v3 v = vadd(vi, delta_v);
+    v3 x = vadd(xi, smul(a.dt, delta_v));
+    x = a.bounds.clamp(x);
+    st3(a.v_pred, i, v);
+    st3(a.x_pred, i, x);
+    a.pos_next[s] = make_float4(x.x, x.y, x.z, xi4.w);
+}
+
+struct PbfLambdaArgs {
+    PassCommon c;
+    const float* __restrict__ rest_density;
+    float lambda_epsilon;
+    int min_neighbors;
+    float* pbf_lambda;
+    float* rho;
+    int* neighbor_count;
+    float4* sca;
+};
+
+// compute_pbf_lambda  solver_pbf.py:245-302
+__global__ __launch_bounds__(kPassBlock) void k_pbf_lambda(PbfLambdaArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    if (!(a.c.sflags[s] & kDynamic)) {
+        a.pbf_lambda[i] = 0.0f;
+        a.rho[i] = 0.0f;
+        a.neighbor_count[i] = 0;
+        a.sca[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float rho0_i = fmax_(a.rest_density[i], 1.0e-6f);
+    const v3 xi = xyz(a.c.pos[s]);
+    const float h = a.c.h;
+    float rho_i = 0.0f;
+    v3 grad_i = V(0.0f, 0.0f, 0.0f);
+    float sum_grad_sq = 0.0f;
+    int count = 0;
+    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
+                                            [&](int k) {
+                                                if (!(a.c.sflags[k] & kActive)) return;
+                                                const float4 xj = a.c.pos[k];
+                                                v3 xij = vsub(xi, xyz(xj));
+                                                float r = vlen(xij);
+                                                if (k != s && r < h) count += 1;
+                                                float mass_j = xj.w;
+                                                rho_i += mass_j * poly6_w(r, h);
+                                                v3 grad_j = smul(-(mass_j / rho0_i), spiky_grad(xij, h));
+                                                sum_grad_sq += vdot(grad_j, grad_j);
+                                                grad_i = vsub(grad_i, grad_j);
+                                            });
+    sum_grad_sq += vdot(grad_i, grad_i);
+    const float c_i = fmax_(rho_i / rho0_i - 1.0f, 0.0f);
+    a.rho[i] = rho_i;
+    a.neighbor_count[i] = count;
+    float lam;
+    if (a.min_neighbors > 0 && count < a.min_neighbors) lam = 0.0f;
+    else lam = -c_i / (sum_grad_sq + a.lambda_epsilon);
+    a.pbf_lambda[i] = lam;
+    a.sca[s] = make_float4(lam, 0.0f, 0.0f, 0.0f);
+}
+
+struct PbfDeltaArgs {
+    PassCommon c;
+    const float4* __restrict__ sca;
+    const float* __restrict__ rest_density;
+    float scorr_k, scorr_n, scorr_w_delta_q, max_delta_position;
+    int apply_bounds;
+    PbfBounds bounds;
+    float* delta_p;
+    float* x_pred;
+    float4* pos_next;
+};
+
+// compute_delta_position + apply_delta_position (+ apply_position_bounds after
+// the last iteration)  solver_pbf.py:305-363, 860-872
+__global__ __launch_bounds__(kPassBlock) void k_pbf_delta(PbfDeltaArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    const float4 xi4 = a.c.pos[s];
+    if (!(a.c.sflags[s] & kDynamic)) {
+        st3(a.delta_p, i, V(0.0f, 0.0f, 0.0f));
+        a.pos_next[s] = xi4;
+        st3(a.x_pred, i, xyz(xi4));
+        return;
+    }
+    const v3 xi = xyz(xi4);
+    const float rho0_i = fmax_(a.rest_density[i], 1.0e-6f);
+    const float lam_i = a.sca[s].x;
+    const float h = a.c.h;
+    v3 dp = V(0.0f, 0.0f, 0.0f);
+    for_each_neighbor<kPassBlock, kListCap>(
+        a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{}, [&](int k) {
+            if (k == s || !(a.c.sflags[k] & kActive)) return;
+            const float4 xj = a.c.pos[k];
+            v3 xij = vsub(xi, xyz(xj));
+            v3 grad = spiky_grad(xij, h);
+            float scorr = 0.0f;
+            if (a.scorr_k != 0.0f) scorr = compute_scorr(vlen(xij), h, a.scorr_k, a.scorr_n, a.scorr_w_delta_q);
+            dp = vadd(dp, smul((lam_i + a.sca[k].x + scorr) * xj.w, grad));
+        });
+    dp = vdivs(dp, rho0_i);
+    if (a.max_delta_position > 0.0f) {
+        float dp_len = vlen(dp);
+        if (dp_len > a.max_delta_position) dp = muls(dp, a.max_delta_position / fmax_(dp_len, 1.0e-12f));
+    }
+    st3(a.delta_p, i, dp);
+    v3 x = vadd(xi, dp);
+    if (a.apply_bounds) x = a.bounds.clamp(x);
+    st3(a.x_pred, i, x);
+    a.pos_next[s] = make_float4(x.x, x.y, x.z, xi4.w);
+}
+
+struct PbfXsphArgs {
+    PassCommon c;
+    const float4* __restrict__ vel;
+    const float4* __restrict__ sca;
+    float xsph_c;
+    float damping;
+    PbfBounds bounds;
+    float* x_out;
+    float* v_out;
+};
+
+// apply_xsph_viscosity + copies + apply_bounds  solver_pbf.py:434-463, 919-954
+__global__ __launch_bounds__(kPassBlock) void k_pbf_xsph(PbfXsphArgs a)
+{
+    __shared__ int lds[kPassBlock * kListCap];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const int i = a.c.perm[s];
+    v3 x = xyz(a.c.pos[s]);
+    const v3 vi = xyz(a.vel[s]);
+    if (!(a.c.sflags[s] & kDynamic)) {
+        st3(a.x_out, i, x);
+        st3(a.v_out, i, vi);
+        return;
+    }
+    const float h = a.c.h;
+    v3 corr = V(0.0f, 0.0f, 0.0f);
+    for_each_neighbor<kPassBlock, kListCap>(
+        a.c.g, a.c.pos, x.x, x.y, x.z, h, a.c.cut2, lds, NoCount{}, [&](int k) {
+            if (k == s || !(a.c.sflags[k] & kActive)) return;
+            const float4 xj = a.c.pos[k];
+            float w = poly6_w(vlen(vsub(x, xyz(xj))), h);
+            corr = vadd(corr, muls(smul(xj.w / fmax_(a.sca[k].x, 1.0e-6f), vsub(xyz(a.vel[k]), vi)), w));
+        });
+    v3 v = vadd(vi, smul(a.xsph_c, corr));
+    a.bounds.clamp_damp(x, v, a.damping);
+    st3(a.x_out, i, x);
+    st3(a.v_out, i, v);
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+
+static PassCommon make_common(const fs_grid* g, const fs_work* w, const float4* pos, float h)
+{
+    PassCommon c;
+    c.g = grid_view(g);
+    c.n = g->num_points;
+    c.perm = g->point_ids;
+    c.pos = pos;
+    c.sflags = w->flags;
+    c.h = h;
+    c.cut2 = h * h * kCutSlack;
+    return c;
+}
+
+static int32_t check_common(const fs_grid* g, const fs_work* w, const fs_model* m)
+{
+    if (!g || !w || !m) {
+        set_error("null grid, workspace or model");
+        return FS_ERR_ARG;
+    }
+    if (m->n < 0 || m->n > g->capacity || m->n > w->capacity) {
+        set_error("particle count %d exceeds grid capacity %d / workspace capacity %d", m->n, g->capacity,
+                  w->capacity);
+        return FS_ERR_CAPACITY;
+    }
+    if (m->n > 0 && (!m->flags || !m->mask || !m->world || !m->mass || !m->gravity || !m->rest_density ||
+                     !m->surface_tension || !m->viscosity || m->num_worlds < 1)) {
+        set_error("model arrays missing");
+        return FS_ERR_ARG;
+    }
+    return FS_OK;
+}
+
+#define LAUNCH_PASS(kernel, nb, s, args)                                                  \
+    do {                                                                                  \
+        hipLaunchKernelGGL(kernel, dim3(nb), dim3(kPassBlock), 0, s, args);               \
+        FS_LAUNCH_CHECK(#kernel);                                                         \
+    } while (0)
+
+static GatherOp gather_for(const fs_model* m, fs_work* w, const float* x, const float* v, float4* vel_dst)
+{
+    GatherOp ga;
+    ga.x = x;
+    ga.v = v;
+    ga.mass = m->mass;
+    ga.rest_density = m->rest_density;
+    ga.flags = m->flags;
+    ga.mask = m->mask;
+    ga.pos = reinterpret_cast<float4*>(w->pos);
+    ga.vel = vel_dst;
+    ga.wvol = w->wvol;
+    ga.sflags = w->flags;
+    return ga;
+}
+
+}  // namespace fs
+
+using namespace fs;
+
+extern "C" {
+
+size_t fs_work_bytes(int32_t capacity)
+{
+    if (capacity < 0) return 0;
+    size_t cap = (size_t)(capacity > 0 ? capacity : 1);
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    size_t nblocks = (cap + kPassBlock - 1) / kPassBlock;
+    return al(cap * 16) * 4 + al(cap * 4) * 2 + al(cap * 12) * 2 + al(nblocks * 16 + 16) + al(16 * 4) + al(8 * 4);
+}
+
+int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity, fs_stream_t stream)
+{
+    if (!w || !workspace || capacity < 0) {
+        set_error("fs_work_init: invalid arguments");
+        return FS_ERR_ARG;
+    }
+    size_t need = fs_work_bytes(capacity);
+    if (bytes < need) {
+        set_error("fs_work_init: workspace %zu bytes < required %zu", bytes, need);
+        return FS_ERR_WORKSPACE;
+    }
+    size_t cap = (size_t)(capacity > 0 ? capacity : 1);
+    size_t nblocks = (cap + kPassBlock - 1) / kPassBlock;
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    char* p = (char*)workspace;
+    auto take = [&](size_t n) {
+        char* r = p;
+        p += al(n);
+        return r;
+    };
+    w->capacity = capacity;
+    w->pos = (float*)take(cap * 16);
+    w->pos_alt = (float*)take(cap * 16);
+    w->vel = (float*)take(cap * 16);
+    w->sca = (float*)take(cap * 16);
+    w->wvol = (float*)take(cap * 4);
+    w->flags = (int32_t*)take(cap * 4);
+    w->x_pred = (float*)take(cap * 12);
+    w->v_pred = (float*)take(cap * 12);
+    w->partials = (double*)take(nblocks * 16 + 16);
+    w->control = (int32_t*)take(16 * 4);
+    w->sync = (uint32_t*)take(8 * 4);
+    hipStream_t s = (hipStream_t)stream;
+    int32_t rc = hip_check(hipMemsetAsync(w->control, 0, 16 * 4, s), "fs_work_init memset");
+    if (rc) return rc;
+    return hip_check(hipMemsetAsync(w->sync, 0, 8 * 4, s), "fs_work_init memset");
+}
+
+int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm, const fs_wcsph_io* io,
+                      fs_stream_t stream)
+{
+    int32_t rc = check_common(g, w, m);
+    if (rc) return rc;
+    if (!prm || !io) {
+        set_error("fs_wcsph_step: null params or io");
+        return FS_ERR_ARG;
+    }
+    const int n = m->n;
+    if (n == 0) return FS_OK;
+    if (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->pressure || !io->accel ||
+        !m->stiffness || !m->exponent) {
+        set_error("fs_wcsph_step: missing arrays");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const float h = prm->smoothing_length;
+    PreOp pre;
+    pre.x_in = io->x_in;
+    rc = build_grid(g, n, h, pre, gather_for(m, w, io->x_in, io->v_in, reinterpret_cast<float4*>(w->vel)), s);
+    if (rc) return rc;
+    const int nb = blocks_for(n, kPassBlock);
+    const float4* pos = reinterpret_cast<const float4*>(w->pos);
+    PassCommon c = make_common(g, w, pos, h);
+
+    WcsphDensityArgs da;
+    da.c = c;
+    da.wvol = w->wvol;
+    da.rest_density = m->rest_density;
+    da.stiffness = m->stiffness;
+    da.exponent = m->exponent;
+    da.clamp = prm->clamp_negative_pressure;
+    da.rho = io->rho;
+    da.pressure = io->pressure;
+    da.sca = reinterpret_cast<float4*>(w->sca);
+    LAUNCH_PASS(k_wcsph_density, nb, s, da);
+
+    WcsphForceArgs fa;
+    fa.c = c;
+    fa.vel = reinterpret_cast<const float4*>(w->vel);
+    fa.sca = reinterpret_cast<const float4*>(w->sca);
+    fa.surface_tension = m->surface_tension;
+    fa.viscosity = m->viscosity;
+    fa.gravity = m->gravity;
+    fa.world = m->world;
+    fa.x_in = io->x_in;
+    fa.v_in = io->v_in;
+    fa.particle_radius = prm->particle_radius;
+    fa.dt = prm->dt;
+    for (int d = 0; d < 3; ++d) fa.bound[d] = prm->bound[d];
+    fa.damping = prm->boundary_damping;
+    fa.accel = io->accel;
+    fa.x_out = io->x_out;
+    fa.v_out = io->v_out;
+    LAUNCH_PASS(k_wcsph_force, nb, s, fa);
+    return FS_OK;
+}
+
+int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm, const fs_dfsph_io* io,
+                      fs_stream_t stream)
+{
+    int32_t rc = check_common(g, w, m);
+    if (rc) return rc;
+    if (!prm || !io) {
+        set_error("fs_dfsph_step: null params or io");
+        return FS_ERR_ARG;
+    }
+    const int n = m->n;
+    if (n == 0) return FS_OK;
+    if (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->factor || !io->rho_star ||
+        !io->d_rho_dt || !io->kappa || !io->kappa_v || !io->neighbor_count || !io->v_star || !io->accel ||
+        (prm->warm_start && (!io->kappa_in || !io->kappa_v_in))) {
+        set_error("fs_dfsph_step: missing arrays");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const float h = prm->smoothing_length;
+    const float dt = prm->dt;
+    const int nb = blocks_for(n, kPassBlock);
+    float4* vstar_s = reinterpret_cast<float4*>(w->pos_alt);
+    float4* sca = reinterpret_cast<float4*>(w->sca);
+
+    // build on x_in (solver_dfsph.py:624)
+    PreOp pre;
+    pre.x_in = io->x_in;
+    pre.control_reset = w->control;
+    rc = build_grid(g, n, h, pre, gather_for(m, w, io->x_in, io->v_in, reinterpret_cast<float4*>(w->vel)), s);
+    if (rc) return rc;
+    PassCommon c = make_common(g, w, reinterpret_cast<const float4*>(w->pos), h);
+
+    DfDensityArgs dfa;
+    dfa.c = c;
+    dfa.wvol = w->wvol;
+    dfa.min_neighbors = prm->min_neighbors_for_factor;
+    dfa.kappa_warm = prm->warm_start ? io->kappa_in : nullptr;
+    dfa.kappa_commit = io->kappa;
+    dfa.rho = io->rho;
+    dfa.factor = io->factor;
+    dfa.neighbor_count = io->neighbor_count;
+    dfa.sca = sca;
+    LAUNCH_PASS(k_dfsph_density_factor, nb, s, dfa);
+
+    DfNonPressureArgs npa;
+    npa.c = c;
+    npa.vel = reinterpret_cast<const float4*>(w->vel);
+    npa.sca = sca;
+    npa.surface_tension = m->surface_tension;
+    npa.viscosity = m->viscosity;
+    npa.gravity = m->gravity;
+    npa.world = m->world;
+    npa.particle_radius = prm->particle_radius;
+    npa.dt = dt;
+    npa.accel = io->accel;
+    npa.v_star = io->v_star;
+    npa.vstar_s = vstar_s;
+    LAUNCH_PASS(k_dfsph_nonpressure, nb, s, npa);
+
+    DfCorrArgs ca;
+    ca.c = c;
+    ca.sca = sca;
+    ca.dt = dt;
+    ca.v_star = io->v_star;
+    ca.vstar_s = vstar_s;
+    ca.control = w->control;
+    if (prm->warm_start) {   // solver_dfsph.py:689-707
+        ca.stop_idx = -1;
+        ca.corr_idx = -1;
+        ca.kappa_commit = nullptr;
+        LAUNCH_PASS(k_dfsph_correction, nb, s, ca);
+    }
+
+    DfRateArgs ra;
+    ra.c = c;
+    ra.vstar_s = vstar_s;
+    ra.rest_density = m->rest_density;
+    ra.dt = dt;
+    ra.d_rho_dt = io->d_rho_dt;
+    ra.rho_star = io->rho_star;
+    ra.sca = sca;
+    ra.partials = w->partials;
+    ra.control = w->control;
+    ra.density_loop = 1;
+    ra.max_kappa = prm->max_kappa;
+    ra.eta = (double)prm->eta_density;
+    ca.stop_idx = kStopRho;
+    ca.corr_idx = kCorrRho;
+    ca.kappa_commit = io->kappa;
+    for (int it = 0; it < prm->max_rho_iterations; ++it) {   // solver_dfsph.py:711-767
+        ra.iter = it;
+        LAUNCH_PASS(k_dfsph_rate, nb, s, ra);
+        LAUNCH_PASS(k_dfsph_correction, nb, s, ca);
+    }
+
+    // integrate_positions fused into the second build's hash (solver_dfsph.py:769-783)
+    PreOp pre2;
+    pre2.mode = kPreIntegrate;
+    pre2.x_in = io->x_in;
+    pre2.v_in = io->v_star;
+    pre2.x_out = io->x_out;
+    pre2.flags = m->flags;
+    pre2.mask = m->mask;
+    pre2.dt = dt;
+    rc = build_grid(g, n, h, pre2, gather_for(m, w, io->x_out, io->v_star, vstar_s), s);
+    if (rc) return rc;
+    c = make_common(g, w, reinterpret_cast<const float4*>(w->pos), h);
+    dfa.c = c;
+    dfa.kappa_warm = prm->warm_start ? io->kappa_v_in : nullptr;
+    dfa.kappa_commit = io->kappa_v;
+    LAUNCH_PASS(k_dfsph_density_factor, nb, s, dfa);   // solver_dfsph.py:785-812
+
+    ca.c = c;
+    if (prm->warm_start) {   // solver_dfsph.py:814-832
+        ca.stop_idx = -1;
+        ca.corr_idx = -1;
+        ca.kappa_commit = nullptr;
+        LAUNCH_PASS(k_dfsph_correction, nb, s, ca);
+    }
+    ra.c = c;
+    ra.density_loop = 0;
+    ra.max_kappa = prm->max_kappa_v;
+    ra.eta = (double)prm->eta_divergence;
+    ca.stop_idx = kStopVel;
+    ca.corr_idx = kCorrVel;
+    ca.kappa_commit = io->kappa_v;
+    for (int it = 0; it < prm->max_vel_iterations; ++it) {   // solver_dfsph.py:834-883
+        ra.iter = it;
+        LAUNCH_PASS(k_dfsph_rate, nb, s, ra);
+        LAUNCH_PASS(k_dfsph_correction, nb, s, ca);
+    }
+
+    DfFinalArgs fa;
+    fa.n = n;
+    fa.flags = m->flags;
+    fa.mask = m->mask;
+    fa.v_star = io->v_star;
+    fa.x_out = io->x_out;
+    fa.v_out = io->v_out;
+    for (int d = 0; d < 3; ++d) fa.bound[d] = prm->bound[d];
+    fa.damping = prm->boundary_damping;
+    fa.control = w->control;
+    fa.iterations = io->iterations;
+    hipLaunchKernelGGL(k_dfsph_finalize, dim3(blocks_for(n, kElemBlock)), dim3(kElemBlock), 0, s, fa);
+    FS_LAUNCH_CHECK("k_dfsph_finalize");
+    return FS_OK;
+}
+
+int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,
+                    fs_stream_t stream)
+{
+    int32_t rc = check_common(g, w, m);
+    if (rc) return rc;
+    if (!prm || !io) {
+        set_error("fs_pbf_step: null params or io");
+        return FS_ERR_ARG;
+    }
+    const int n = m->n;
+    if (n == 0) return FS_OK;
+    if (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->pbf_lambda || !io->neighbor_count ||
+        !io->delta_p || !io->accel) {
+        set_error("fs_pbf_step: missing arrays");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const float h = prm->smoothing_length;
+    const float dt = prm->dt;
+    const int nb = blocks_for(n, kPassBlock);
+    float4* sca = reinterpret_cast<float4*>(w->sca);
+    float4* bufs[2] = {reinterpret_cast<float4*>(w->pos), reinterpret_cast<float4*>(w->pos_alt)};
+    int cur = 0;
+    PbfBounds bounds;
+    for (int d = 0; d < 3; ++d) bounds.bound[d] = prm->bound[d];
+    bounds.particle_radius = prm->particle_radius;
+    bounds.boundary_epsilon = prm->boundary_epsilon;
+
+    // predict + bounds fused into the build's hash (solver_pbf.py:702-737)
+    PreOp pre;
+    pre.mode = kPrePredict;
+    pre.x_in = io->x_in;
+    pre.v_in = io->v_in;
+    pre.x_out = w->x_pred;
+    pre.v_out = w->v_pred;
+    pre.flags = m->flags;
+    pre.mask = m->mask;
+    pre.world = m->world;
+    pre.gravity = m->gravity;
+    pre.dt = dt;
+    for (int d = 0; d < 3; ++d) pre.bound[d] = prm->bound[d];
+    pre.particle_radius = prm->particle_radius;
+    pre.boundary_epsilon = prm->boundary_epsilon;
+    rc = build_grid(g, n, h, pre, gather_for(m, w, w->x_pred, w->v_pred, reinterpret_cast<float4*>(w->vel)), s);
+    if (rc) return rc;
+
+    PbfDensityArgs da;
+    da.c = make_common(g, w, bufs[cur], h);
+    da.wvol = w->wvol;
+    da.rho = io->rho;
+    da.sca = sca;
+    da.final_pass = 0;
+    da.finalize = 0;
+    da.x_in = io->x_in;
+    da.v_in = io->v_in;
+    da.dt = dt;
+    da.damping = prm->boundary_damping;
+    da.bounds = bounds;
+    da.vel = reinterpret_cast<float4*>(w->vel);
+    da.x_out = io->x_out;
+    da.v_out = io->v_out;
+    LAUNCH_PASS(k_pbf_density, nb, s, da);   // :738-751
+
+    PbfNonPressureArgs na;
+    na.c = make_common(g, w, bufs[cur], h);
+    na.vel = reinterpret_cast<const float4*>(w->vel);
+    na.sca = sca;
+    na.surface_tension = m->surface_tension;
+    na.viscosity = m->viscosity;
+    na.particle_radius = prm->particle_radius;
+    na.dt = dt;
+    na.bounds = bounds;
+    na.accel = io->accel;
+    na.x_pred = w->x_pred;
+    na.v_pred = w->v_pred;
+    na.pos_next = bufs[cur ^ 1];
+    LAUNCH_PASS(k_pbf_nonpressure, nb, s, na);   // :752-796
+    cur ^= 1;
+
+    for (int it = 0; it < prm->max_iterations; ++it) {   // :800-858
+        PbfLambdaArgs la;
+        la.c = make_common(g, w, bufs[cur], h);
+        la.rest_density = m->rest_density;
+        la.lambda_epsilon = prm->lambda_epsilon;
+        la.min_neighbors = prm->min_neighbors_for_lambda;
+        la.pbf_lambda = io->pbf_lambda;
+        la.rho = io->rho;
+        la.neighbor_count = io->neighbor_count;
+        la.sca = sca;
+        LAUNCH_PASS(k_pbf_lambda, nb, s, la);
+
+        PbfDeltaArgs pa;
+        pa.c = make_common(g, w, bufs[cur], h);
+        pa.sca = sca;
+        pa.rest_density = m->rest_density;
+        pa.scorr_k = prm->scorr_k;
+        pa.scorr_n = prm->scorr_n;
+        pa.scorr_w_delta_q = prm->scorr_w_delta_q;
+        pa.max_delta_position = prm->max_delta_position;
+        pa.apply_bounds = (it == prm->max_iterations - 1 || prm->rebuild_grid_each_iteration) ? 1 : 0;
+        pa.bounds = bounds;
+        pa.delta_p = io->delta_p;
+        pa.x_pred = w->x_pred;
+        pa.pos_next = bufs[cur ^ 1];
+        LAUNCH_PASS(k_pbf_delta, nb, s, pa);
+        cur ^= 1;
+        if (prm->rebuild_grid_each_iteration) {
+            PreOp rp;
+            rp.x_in = w->x_pred;
+            rc = build_grid(g, n, h, rp, gather_for(m, w, w->x_pred, nullptr, nullptr), s);
+            if (rc) return rc;
+            cur = 0;   // the gather writes w->pos
+        }
+    }
+
+    const bool xsph = prm->xsph_c > 0.0f;
+    da.c = make_common(g, w, bufs[cur], h);
+    da.final_pass = 1;
+    da.finalize = xsph ? 0 : 1;
+    LAUNCH_PASS(k_pbf_density, nb, s, da);   // :878-917 (+ :939-954 without XSPH)
+    if (xsph) {
+        PbfXsphArgs xa;
+        xa.c = make_common(g, w, bufs[cur], h);
+        xa.vel = reinterpret_cast<const float4*>(w->vel);
+        xa.sca = sca;
+        xa.xsph_c = prm->xsph_c;
+        xa.damping = prm->boundary_damping;
+        xa.bounds = bounds;
+        xa.x_out = io->x_out;
+        xa.v_out = io->v_out;
+        LAUNCH_PASS(k_pbf_xsph, nb, s, xa);   // :919-954
+    }
+    return FS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,288 @@
+/*
+ * fluidsim.h -- C-ABI of libfluidsim_hip.so, the MI355X (gfx950) particle-fluid
+ * hot path: Warp-compatible hash-grid build and query, and the neighbour-loop
+ * passes of the WCSPH, DFSPH and PBF solvers.
+ *
+ * Conventions
+ *   - Every pointer argument named in an *_io / fs_model struct is a DEVICE
+ *     pointer owned by the caller (e.g. torch.Tensor.data_ptr()).  Vectors are
+ *     AoS float32 xyz with stride 12 bytes (Warp vec3 / torch (N,3)).
+ *   - Workspaces are caller-allocated device memory sized by *_bytes().  No
+ *     entry point allocates device memory; none synchronises the host except
+ *     the ones whose name ends in _readback.
+ *   - fs_stream_t is a hipStream_t (NULL = default stream).  Work is enqueued
+ *     in order on that stream.
+ *   - Return value: FS_OK (0) or a negative FS_ERR_* code; fs_last_error()
+ *     then describes the failure (thread-local).
+ *
+ * Reference interfaces replaced (SParksLz/fluid_simulation):
+ *   - warp HashGrid(128,128,128).reserve/build and hash_grid_query, as used by
+ *     solver/sph/solver_wcsph.py:398-402,456; solver/sph/solver_dfsph.py:541-545,
+ *     624,783; solver/pbf/solver_pbf.py:634-638,737,858.
+ *   - SolverWCSPH.step   solver/sph/solver_wcsph.py:431-535
+ *   - SolverDFSPH.step   solver/sph/solver_dfsph.py:598-898
+ *   - SolverPBF.step     solver/pbf/solver_pbf.py:676-954
+ *   - particle_norm_maxima / check_particle_finite
+ *                        examples/dam_break/simulation.py:142-176
+ */
+#ifndef FLUIDSIM_H
+#define FLUIDSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* fs_stream_t;
+
+enum {
+    FS_OK = 0,
+    FS_ERR_ARG = -1,        /* bad argument (null pointer, negative size, ...) */
+    FS_ERR_CAPACITY = -2,   /* more points than the grid / workspace reserved  */
+    FS_ERR_HIP = -3,        /* a HIP runtime call failed                       */
+    FS_ERR_WORKSPACE = -4   /* workspace too small                             */
+};
+
+/* Description of the last error on the calling thread ("" if none). */
+const char* fs_last_error(void);
+/* Library ABI version (major*100 + minor). */
+int32_t fs_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Hash grid                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* Device-resident uniform hash grid with Warp HashGrid semantics:
+ *   bucket of a point  = hash(round(x/w), round(y/w), round(z/w)) with
+ *                        hash(c) = max(c + 2^20, 0) mod dim per axis and
+ *                        id = bz*dim_x*dim_y + by*dim_x + bx;
+ *   sorted order       = stable by (bucket, original index);
+ *   query(p, r)        = buckets of cells round((p-r)/w)..round((p+r)/w) per
+ *                        axis (at most dim per axis), x fastest, then y, z.
+ * Fields are filled by fs_grid_init / fs_grid_build; read them, do not write. */
+typedef struct fs_grid {
+    int32_t dim_x, dim_y, dim_z;
+    int32_t round_mode;     /* 0 = floor (default), 1 = truncate toward zero  */
+    int32_t capacity;       /* points reserved                                 */
+    int32_t num_points;     /* points in the last build                        */
+    float cell_width;       /* radius of the last build                        */
+    float cell_width_inv;   /* 1/cell_width in float32                         */
+    int32_t* point_ids;     /* [capacity] sorted slot -> original index        */
+    int32_t* point_cells;   /* [capacity] sorted bucket keys                   */
+    int32_t* cell_of;       /* [capacity] bucket of original point i           */
+    int32_t* cell_start;    /* [dims+1] exclusive prefix over buckets           */
+    int32_t* cell_count;    /* [dims] per-bucket counters, zero between builds  */
+    int32_t* rank;          /* [capacity] scratch                              */
+    int32_t* scatter_ids;   /* [capacity] scratch                              */
+    uint64_t* scan_state;   /* look-back descriptors of the bucket scan         */
+    uint32_t* sync;         /* [8] self-resetting counters                      */
+} fs_grid;
+
+/* Bytes of device workspace for a grid reserving `capacity` points. */
+size_t fs_grid_workspace_bytes(int32_t capacity, int32_t dim_x, int32_t dim_y, int32_t dim_z);
+
+/* Carve `workspace` (device, >= fs_grid_workspace_bytes) into the grid arrays
+ * and zero the counters on `stream`.  Warp HashGrid(dim_x,dim_y,dim_z) +
+ * reserve(capacity)  (solver_wcsph.py:398-402). */
+int32_t fs_grid_init(fs_grid* grid, void* workspace, size_t workspace_bytes, int32_t capacity, int32_t dim_x,
+                     int32_t dim_y, int32_t dim_z, int32_t round_mode, fs_stream_t stream);
+
+/* Build from `n` AoS float3 points with cell width `radius`.
+ * Warp HashGrid.build(points, radius)  (solver_wcsph.py:456). */
+int32_t fs_grid_build(fs_grid* grid, const float* points, int32_t n, float radius, fs_stream_t stream);
+
+/* Warp-layout bucket tables of the last build: cell_starts/cell_ends [dims],
+ * 0/0 for empty buckets (as Warp's build leaves them). */
+int32_t fs_grid_export_tables(const fs_grid* grid, int32_t* cell_starts, int32_t* cell_ends, fs_stream_t stream);
+
+/* Per-point candidate counts (counts[i] for original point i) of
+ * hash_grid_query(points[i], radius) on the last build.
+ * within_mode: -1 all candidates (self and aliased buckets included),
+ *               0 cubic-spline support  fl(sqrt(d2))/radius <= 1,
+ *               1 PBF support           fl(sqrt(d2)) < radius. */
+int32_t fs_grid_count_candidates(const fs_grid* grid, const float* points, float radius, int32_t within_mode,
+                                 int32_t* counts, fs_stream_t stream);
+
+/* Candidate indices (original numbering) in query visit order, CSR layout:
+ * indices[offsets[i] .. offsets[i+1]) for point i.  offsets: int64 [n+1]
+ * (e.g. an exclusive scan of fs_grid_count_candidates). */
+int32_t fs_grid_list_candidates(const fs_grid* grid, const float* points, float radius, int32_t within_mode,
+                                const int64_t* offsets, int32_t* indices, fs_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Solver workspace and model                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Per-particle model arrays in original order (Newton Model + sph: MODEL
+ * attributes, solver_wcsph.py:320-392).  stiffness/exponent are read by
+ * WCSPH only and may be NULL otherwise. */
+typedef struct fs_model {
+    int32_t n;
+    int32_t num_worlds;
+    const int32_t* flags;          /* ParticleFlags, bit 0 = ACTIVE            */
+    const int32_t* mask;           /* sph:particle_mask                       */
+    const int32_t* world;          /* particle_world                          */
+    const float* mass;
+    const float* gravity;          /* [num_worlds*3]                          */
+    const float* rest_density;
+    const float* surface_tension;
+    const float* viscosity;
+    const float* stiffness;
+    const float* exponent;
+} fs_model;
+
+/* Solver scratch carved from one caller-allocated workspace: particle data
+ * gathered into the grid's sorted order, plus reduction scratch. */
+typedef struct fs_work {
+    int32_t capacity;
+    float* pos;         /* [cap*4] x,y,z,mass  sorted                          */
+    float* pos_alt;     /* [cap*4] ping-pong positions (PBF)                    */
+    float* vel;         /* [cap*4] vx,vy,vz,-  sorted                          */
+    float* sca;         /* [cap*4] neighbour-visible per-pass scalars, sorted  */
+    float* wvol;        /* [cap]   rest_density*volume, sorted                 */
+    int32_t* flags;     /* [cap]   flags (bit0 active, bit1 dynamic), sorted   */
+    float* x_pred;      /* [cap*3] original order (PBF x_pred / DFSPH scratch)  */
+    float* v_pred;      /* [cap*3] original order (PBF v_pred)                 */
+    double* partials;   /* residual partial sums                               */
+    int32_t* control;   /* [16] loop control: stop flags, iteration counters   */
+    uint32_t* sync;     /* [8]  self-resetting counters                        */
+} fs_work;
+
+size_t fs_work_bytes(int32_t capacity);
+int32_t fs_work_init(fs_work* work, void* workspace, size_t workspace_bytes, int32_t capacity, fs_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* WCSPH                                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fs_wcsph_params {
+    float smoothing_length;        /* h = particle_length * coff             */
+    float particle_radius;         /* e_dist floor in the cohesion term      */
+    float bound[3];                /* bound_width, bound_height, bound_length */
+    float boundary_damping;
+    float dt;
+    int32_t clamp_negative_pressure;
+} fs_wcsph_params;
+
+typedef struct fs_wcsph_io {
+    const float* x_in;   /* state_in.particle_q   */
+    const float* v_in;   /* state_in.particle_qd  */
+    float* x_out;        /* state_out.particle_q  */
+    float* v_out;        /* state_out.particle_qd */
+    float* rho;          /* state_out.sph.rho     */
+    float* pressure;     /* state_out.sph.pressure */
+    float* accel;        /* solver._particle_accel [n*3] */
+} fs_wcsph_io;
+
+/* One WCSPH substep: build, compute_density_pressure, acceleration_wcsph,
+ * integrate_velocity, integrate_positions, apply_bounds
+ * (SolverWCSPH.step, solver_wcsph.py:431-535). */
+int32_t fs_wcsph_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_wcsph_params* params,
+                      const fs_wcsph_io* io, fs_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* DFSPH                                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fs_dfsph_params {
+    double eta_density;        /* compared in float64, as the reference's   */
+    double eta_divergence;     /* Python-side residual test (:733, :850)    */
+    float smoothing_length;
+    float particle_radius;
+    float bound[3];
+    float boundary_damping;
+    float dt;
+    float max_kappa;
+    float max_kappa_v;
+    int32_t max_rho_iterations;
+    int32_t max_vel_iterations;
+    int32_t warm_start;
+    int32_t min_neighbors_for_factor;
+} fs_dfsph_params;
+
+typedef struct fs_dfsph_io {
+    const float* x_in;
+    const float* v_in;
+    const float* kappa_in;     /* state_in.sph.kappa   (warm start)  */
+    const float* kappa_v_in;   /* state_in.sph.kappa_v (warm start)  */
+    float* x_out;
+    float* v_out;
+    float* rho;
+    float* factor;
+    float* rho_star;
+    float* d_rho_dt;
+    float* kappa;
+    float* kappa_v;
+    int32_t* neighbor_count;
+    float* v_star;             /* [n*3] */
+    float* accel;              /* [n*3] solver._particle_accel */
+    int32_t* iterations;       /* device int32[4]: density iterations,
+                                  density corrections, divergence iterations,
+                                  divergence corrections (may be NULL) */
+} fs_dfsph_io;
+
+/* One DFSPH substep (SolverDFSPH.step, solver_dfsph.py:598-898).  The
+ * residual early exits (:733-735, :849-851) are decided on the device:
+ * residual = mean over dynamic particles of max(x, 0), accumulated in
+ * float64; the remaining passes of a finished loop return at once. */
+int32_t fs_dfsph_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_dfsph_params* params,
+                      const fs_dfsph_io* io, fs_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* PBF                                                                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct fs_pbf_params {
+    float smoothing_length;
+    float particle_radius;
+    float bound[3];
+    float boundary_damping;
+    float boundary_epsilon;
+    float dt;
+    float lambda_epsilon;
+    float scorr_k;
+    float scorr_n;
+    float scorr_w_delta_q;     /* poly6(scorr_q*h)  (solver_pbf.py:671-674) */
+    float max_delta_position;
+    float xsph_c;
+    int32_t max_iterations;
+    int32_t min_neighbors_for_lambda;
+    int32_t rebuild_grid_each_iteration;
+} fs_pbf_params;
+
+typedef struct fs_pbf_io {
+    const float* x_in;
+    const float* v_in;
+    float* x_out;
+    float* v_out;
+    float* rho;
+    float* pbf_lambda;
+    int32_t* neighbor_count;
+    float* delta_p;            /* [n*3] */
+    float* accel;              /* [n*3] solver._particle_accel */
+} fs_pbf_io;
+
+/* One PBF substep (SolverPBF.step, solver_pbf.py:676-954). */
+int32_t fs_pbf_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_pbf_params* params,
+                    const fs_pbf_io* io, fs_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Driver diagnostics  (examples/dam_break/simulation.py:142-176)           */
+/* ------------------------------------------------------------------------ */
+
+/* maxima[0] = max |v|^2 over nv points, maxima[1] = max |a|^2 over na points
+ * (non-finite -> +inf); maxima is device float[2], overwritten. */
+int32_t fs_norm_maxima(const float* v, int32_t nv, const float* a, int32_t na, float* maxima, fs_stream_t stream);
+
+/* invalid[0] = 1 if any x or v component (or density, first nd) is
+ * non-finite, else 0; invalid is device int32[1], overwritten. */
+int32_t fs_check_finite(const float* x, const float* v, int32_t n, const float* density, int32_t nd,
+                        int32_t* invalid, fs_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUIDSIM_H */
