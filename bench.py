@@ -1,0 +1,200 @@
+"""Benchmark: particle-substeps/s of the 401,856-particle dam break on one GPU.
+
+One "step" is one solver substep (``Solver*.step``: grid build + every
+neighbour pass of that solver) over the full dam-break scene
+(``make_scene(400000)`` -> 401,856 particles), inputs resident in HBM.  The
+headline ``value`` is WCSPH (the solver the reference publishes a substep rate
+for); DFSPH and PBF are measured the same way and reported under
+``solvers``.  dt follows the dam-break driver's policy, evaluated once on the
+device state before the timed region so the timed substeps run back to back
+without host synchronisation.
+
+``roofline`` covers the dominant kernel of the WCSPH substep: algorithmic bytes
+per launch (the per-particle byte model below x 401,856) over its mean launch
+duration, from HIP events on the launch stream.  ``cpu_baseline`` times the
+oracle's FP32 restatement of the same WCSPH substep on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--solvers wcsph,dfsph,pbf]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PUBLISHED_WCSPH = 62.78e6     # particle-substeps/s derived from the reference's WCSPH run (RTX 4090)
+
+# Algorithmic bytes per particle of each kernel (one launch processes every
+# particle once): the pass byte model of the hot path -- each pass reads every
+# per-particle array it uses once and writes each result once; neighbour
+# gathers are on-chip reuse and cost nothing.
+PASS_BYTES = {
+    "k_wcsph_density": 44, "k_wcsph_force": 96,
+    "k_dfsph_density_factor": 40, "k_dfsph_nonpressure": 68, "k_dfsph_rate": 64, "k_dfsph_correction": 60,
+    "k_pbf_density": 32, "k_pbf_nonpressure": 76, "k_pbf_lambda": 44, "k_pbf_delta": 72, "k_pbf_xsph": 80,
+}
+# bytes per particle-substep of each solver (grid build counted at 28 B)
+SOLVER_BYTES = {"wcsph": 168, "pbf": 760}
+
+
+def dfsph_bytes(rho_it, rho_corr, vel_it, vel_corr):
+    return 260 + 64 * rho_it + 60 * rho_corr + 56 * vel_it + 60 * vel_corr
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--particles", type=int, default=400000)
+    ap.add_argument("--solvers", default="wcsph,dfsph,pbf")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-substeps", type=int, default=40)
+    return ap.parse_args()
+
+
+def driver_dt(sim):
+    if sim.method == "pbf":
+        return sim.frame_dt / sim.substeps
+    return min(sim.frame_dt / sim.substeps, sim.cfl_limit(*sim._cfl_maxima()))
+
+
+def bench_solver(method, scene, steps, warmup):
+    import torch
+
+    from fluid_simulation_amd import _lib
+    from fluid_simulation_amd.examples.dam_break.simulation import DamBreakSimulation
+
+    sim = DamBreakSimulation(scene, method, device="cuda:0")
+    for _ in range(warmup):
+        sim._substep(driver_dt(sim))
+    dt = driver_dt(sim)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim._substep(dt)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    iters = sim.solver.last_iterations() if method == "dfsph" else None
+    # per-kernel timing: the same substeps again with HIP events around each launch
+    lib = _lib.load()
+    lib.fs_profile_reset()
+    lib.fs_profile_enable(1)
+    for _ in range(steps):
+        sim._substep(dt)
+    lib.fs_profile_enable(0)
+    prof = _lib.profile_report()
+    sim.check_finite()
+    n = scene.count
+    rate = n * steps / elapsed
+    out = {"particle_substeps_per_s": rate, "ms_per_substep": 1e3 * elapsed / steps, "dt": dt,
+           "kernels_ms_per_substep": {k: v[1] / steps for k, v in sorted(prof.items())}}
+    if method == "dfsph":
+        out["iterations_last_substep"] = iters
+        b = dfsph_bytes(iters["density_iterations"], iters["density_corrections"], iters["divergence_iterations"],
+                        iters["divergence_corrections"])
+    else:
+        b = SOLVER_BYTES[method]
+    out["bytes_per_particle_substep"] = b
+    out["effective_GBs"] = b * rate / 1e9
+    out["hbm_frac"] = out["effective_GBs"] / HBM_PEAK_GBS
+    return out, prof
+
+
+def cpu_baseline(scene, substeps):
+    import numpy as np
+
+    from oracle import oracle as O
+    from fluid_simulation_amd.examples.dam_break.simulation import material, solver_config
+
+    threads = min(os.cpu_count() or 1, 16)
+    O.set_threads(threads)
+    cfg = solver_config("wcsph", scene)
+    mat = material("wcsph", scene)
+    n = scene.count
+    m = O.uniform_model(n, scene.mass, stiffness=mat["stiffness"], exponent=mat["exponent"],
+                        viscosity=mat["viscosity"], surface_tension=mat["surface_tension"],
+                        gravity=scene.spec.gravity, radius=scene.radius)
+    m.world[:] = -1
+    st = O.OState(scene.positions.copy(), np.zeros_like(scene.positions))
+    ocfg = O.WcsphConfig(**{k: getattr(cfg, k) for k in O.WcsphConfig.__dataclass_fields__})
+    dt = 1.0e-4
+    O.wcsph_step(m, st, ocfg, dt)   # warm-up (page-in, thread pool)
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        st, _ = O.wcsph_step(m, st, ocfg, dt)
+    el = time.perf_counter() - t0
+    return {"value": n * substeps / el, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"{substeps} WCSPH substeps of the {n}-particle dam break, FP32 C oracle, OpenMP "
+                      f"{threads} threads, {el:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path does not shard "
+              "across GPUs here)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from fluid_simulation_amd.examples.dam_break.scene import make_scene
+
+    scene = make_scene(args.particles)
+    results = {}
+    prof_wcsph = None
+    for method in [s for s in args.solvers.split(",") if s]:
+        print(f"# bench {method}: n={scene.count} steps={args.steps} warmup={args.warmup}", file=sys.stderr,
+              flush=True)
+        res, prof = bench_solver(method, scene, args.steps, args.warmup)
+        results[method] = res
+        if method == "wcsph":
+            prof_wcsph = prof
+        print(f"# {method}: {res['particle_substeps_per_s'] / 1e6:.1f} M particle-substeps/s, "
+              f"{res['ms_per_substep']:.3f} ms/substep", file=sys.stderr, flush=True)
+    head = "wcsph" if "wcsph" in results else next(iter(results))
+    headline = results[head]
+    prof = prof_wcsph if prof_wcsph is not None else prof
+    dom = max((k for k in prof if k in PASS_BYTES), key=lambda k: prof[k][1])
+    launches, total_ms = prof[dom]
+    avg_s = total_ms / launches / 1e3
+    achieved = PASS_BYTES[dom] * scene.count / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": avg_s * 1e6,
+                "bytes_per_launch": PASS_BYTES[dom] * scene.count}
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("# cpu baseline (oracle)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(scene, args.cpu_substeps)
+    value = headline["particle_substeps_per_s"]
+    line = {
+        "metric": "particle-substeps/s @402k dam-break, % HBM peak; batched & adjoint vs plain",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": headline["ms_per_substep"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_WCSPH if head == "wcsph" else None,
+        "dtype": "f32",
+        "data": "synthetic: make_scene(400000) dam-break lattice (401,856 particles), zero initial velocity",
+        "config": {"workload": f"{head} dam break, {scene.count} particles, one solver substep per step",
+                   "particles": scene.count, "solver": head, "dt": headline["dt"]},
+        "hbm_frac_substep": headline["hbm_frac"],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "solvers": results,
+        "device": torch.cuda.get_device_name(0),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

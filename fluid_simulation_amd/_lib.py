@@ -100,6 +100,9 @@ class FsPbfIO(ctypes.Structure):
 SIGNATURES = {
     "fs_last_error": (ctypes.c_char_p, []),
     "fs_version": (c_int32, []),
+    "fs_profile_enable": (None, [c_int32]),
+    "fs_profile_reset": (None, []),
+    "fs_profile_report": (c_int32, [ctypes.c_char_p, c_size_t]),
     "fs_grid_workspace_bytes": (c_size_t, [c_int32, c_int32, c_int32, c_int32]),
     "fs_grid_init": (c_int32, [POINTER(FsGrid), c_void_p, c_size_t, c_int32, c_int32, c_int32, c_int32, c_int32,
                                c_void_p]),
@@ -155,6 +158,19 @@ def require_device(device) -> None:
         raise RuntimeError(f"fluid_simulation_amd solvers run on a HIP device only (got device={device!r}); "
                            "there is no CPU fallback")
     load()
+
+
+def profile_report() -> dict:
+    """{kernel: (launches, total_ms)} accumulated while profiling was enabled."""
+    lib = load()
+    n = lib.fs_profile_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    lib.fs_profile_report(buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, count, ms = line.rsplit(" ", 2)
+        out[name] = (int(count), float(ms))
+    return out
 
 
 def stream_handle(device=None) -> int:

@@ -199,6 +199,7 @@ __host__ __device__ __forceinline__ int hash_axis(int c, int dim)
 struct GridView {
     int dx, dy, dz, dxy;
     int round_mode;
+    int npts;                       // points in the build (slot ranges are clamped to it)
     float inv;
     const int* __restrict__ start;  // [dx*dy*dz + 1]
 };
@@ -258,8 +259,8 @@ struct QueryRows {
         row_base = hash_axis(zs + zi, g.dz) * g.dxy + hash_axis(ys + yi, g.dy) * g.dx;
         int a = row_base + (part == 0 ? hx0 : 0);
         int b = a + (part == 0 ? seg1 : seg2);
-        k = g.start[a];
-        kend = g.start[b];
+        k = min(max(g.start[a], 0), g.npts);
+        kend = min(max(g.start[b], k), g.npts);
     }
     // Move to the next row; false when the query is exhausted.
     __device__ __forceinline__ bool advance(const GridView& g)

@@ -14,6 +14,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <map>
+#include <string>
+#include <vector>
+
 #include "fs_internal.h"
 
 namespace fs {
@@ -35,6 +39,54 @@ int32_t hip_check(hipError_t e, const char* what)
     return FS_ERR_HIP;
 }
 
+bool g_profiling = false;
+
+struct ProfRecord {
+    const char* name;
+    hipEvent_t ev[2];
+};
+static std::vector<ProfRecord> g_prof_live;
+static std::vector<hipEvent_t> g_prof_pool;
+static std::map<std::string, std::pair<double, long long>> g_prof_totals;
+
+static hipEvent_t prof_event()
+{
+    if (!g_prof_pool.empty()) {
+        hipEvent_t e = g_prof_pool.back();
+        g_prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void prof_mark(const char* name, hipStream_t s, bool begin)
+{
+    if (begin) {
+        ProfRecord r{name, {prof_event(), prof_event()}};
+        (void)hipEventRecord(r.ev[0], s);
+        g_prof_live.push_back(r);
+    } else if (!g_prof_live.empty()) {
+        (void)hipEventRecord(g_prof_live.back().ev[1], s);
+    }
+}
+
+static void prof_collect()
+{
+    for (auto& r : g_prof_live) {
+        float ms = 0.0f;
+        (void)hipEventSynchronize(r.ev[1]);
+        (void)hipEventElapsedTime(&ms, r.ev[0], r.ev[1]);
+        auto& t = g_prof_totals[r.name];
+        t.first += ms;
+        t.second += 1;
+        g_prof_pool.push_back(r.ev[0]);
+        g_prof_pool.push_back(r.ev[1]);
+    }
+    g_prof_live.clear();
+}
+
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanBlock * kScanItems;
@@ -49,6 +101,7 @@ GridView grid_view(const fs_grid* g)
     v.dz = g->dim_z;
     v.dxy = g->dim_x * g->dim_y;
     v.round_mode = g->round_mode;
+    v.npts = g->num_points;
     v.inv = g->cell_width_inv;
     v.start = g->cell_start;
     return v;
@@ -132,6 +185,7 @@ __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restric
     __syncthreads();
     const int tile = s_tile;
     const uint32_t epoch = s_epoch;
+    if (tile >= ntiles) return;   // only reachable with corrupted counters
     const int base = tile * kScanTile + t * kScanItems;
 
     int v[kScanItems];
@@ -243,7 +297,8 @@ __global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __rest
     const int i = blockIdx.x * kElemBlock + threadIdx.x;
     if (i >= n) return;
     int c = cell_of[i];
-    scatter_ids[start[c] + rank[i]] = i;
+    int slot = start[c] + rank[i];
+    if ((unsigned)slot < (unsigned)n) scatter_ids[slot] = i;
     count[c] = 0;
 }
 
@@ -256,7 +311,7 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int* __r
     if (k >= n) return;
     const int i = scatter_ids[k];
     const int c = cell_of[i];
-    const int a = start[c], b = start[c + 1];
+    const int a = min(max(start[c], 0), n), b = min(max(start[c + 1], a), n);
     int r = 0;
     for (int m = a; m < b; ++m) r += (scatter_ids[m] < i) ? 1 : 0;
     const int f = a + r;
@@ -354,23 +409,19 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
     if (nb > 0 || pre.control_reset) {
         const int nbh = nb > 0 ? nb : 1;
         switch (pre.mode) {
-        case kPreIntegrate: hipLaunchKernelGGL(k_hash_count<kPreIntegrate>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
-        case kPrePredict: hipLaunchKernelGGL(k_hash_count<kPrePredict>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
-        default: hipLaunchKernelGGL(k_hash_count<kPrePlain>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
+        case kPreIntegrate: FS_LAUNCH(k_hash_count<kPreIntegrate>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
+        case kPrePredict: FS_LAUNCH(k_hash_count<kPrePredict>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
+        default: FS_LAUNCH(k_hash_count<kPrePlain>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
         }
-        FS_LAUNCH_CHECK("k_hash_count");
     }
     const int ntiles = scan_tiles(g);
-    hipLaunchKernelGGL(k_bucket_scan, dim3(ntiles), dim3(kScanBlock), 0, s, g->cell_count, g->cell_start, m, ntiles,
-                       g->scan_state, g->sync);
-    FS_LAUNCH_CHECK("k_bucket_scan");
+    FS_LAUNCH(k_bucket_scan, dim3(ntiles), dim3(kScanBlock), 0, s, g->cell_count, g->cell_start, m, ntiles,
+              g->scan_state, g->sync);
     if (nb == 0) return FS_OK;
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(kElemBlock), 0, s, n, g->cell_of, g->rank, g->cell_start,
-                       g->scatter_ids, g->cell_count);
-    FS_LAUNCH_CHECK("k_scatter");
-    hipLaunchKernelGGL(k_fix_gather, dim3(nb), dim3(kElemBlock), 0, s, n, g->scatter_ids, g->cell_of, g->cell_start,
-                       g->point_ids, g->point_cells, ga);
-    FS_LAUNCH_CHECK("k_fix_gather");
+    FS_LAUNCH(k_scatter, dim3(nb), dim3(kElemBlock), 0, s, n, g->cell_of, g->rank, g->cell_start, g->scatter_ids,
+              g->cell_count);
+    FS_LAUNCH(k_fix_gather, dim3(nb), dim3(kElemBlock), 0, s, n, g->scatter_ids, g->cell_of, g->cell_start,
+              g->point_ids, g->point_cells, ga);
     return FS_OK;
 }
 
@@ -381,6 +432,35 @@ using namespace fs;
 extern "C" {
 
 const char* fs_last_error(void) { return g_err; }
+
+void fs_profile_enable(int32_t on)
+{
+    if (!on) prof_collect();
+    g_profiling = on != 0;
+}
+
+void fs_profile_reset(void)
+{
+    prof_collect();
+    g_prof_totals.clear();
+}
+
+int32_t fs_profile_report(char* buf, size_t len)
+{
+    prof_collect();
+    std::string out;
+    for (auto& kv : g_prof_totals) {
+        char line[256];
+        snprintf(line, sizeof(line), "%s %lld %.6f\n", kv.first.c_str(), kv.second.second, kv.second.first);
+        out += line;
+    }
+    if (buf && len) {
+        size_t n = out.size() < len - 1 ? out.size() : len - 1;
+        memcpy(buf, out.data(), n);
+        buf[n] = 0;
+    }
+    return (int32_t)out.size();
+}
 
 int32_t fs_version(void) { return 100; }
 

@@ -53,4 +53,17 @@ GridView grid_view(const fs_grid* g);
 
 inline int blocks_for(int n, int block) { return (n + block - 1) / block; }
 
+// Optional per-launch timing (fs_profile_enable): events recorded on the
+// launch stream around each kernel, summed per kernel name on collection.
+extern bool g_profiling;
+void prof_mark(const char* name, hipStream_t s, bool begin);
+
+#define FS_LAUNCH(kernel, grid, block, shmem, stream, ...)                         \
+    do {                                                                           \
+        if (::fs::g_profiling) ::fs::prof_mark(#kernel, stream, true);             \
+        hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);       \
+        if (::fs::g_profiling) ::fs::prof_mark(#kernel, stream, false);            \
+        FS_LAUNCH_CHECK(#kernel);                                                  \
+    } while (0)
+
 }  // namespace fs

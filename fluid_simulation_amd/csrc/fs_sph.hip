@@ -833,11 +833,7 @@ static int32_t check_common(const fs_grid* g, const fs_work* w, const fs_model* 
     return FS_OK;
 }
 
-#define LAUNCH_PASS(kernel, nb, s, args)                                                  \
-    do {                                                                                  \
-        hipLaunchKernelGGL(kernel, dim3(nb), dim3(kPassBlock), 0, s, args);               \
-        FS_LAUNCH_CHECK(#kernel);                                                         \
-    } while (0)
+#define LAUNCH_PASS(kernel, nb, s, args) FS_LAUNCH(kernel, dim3(nb), dim3(kPassBlock), 0, s, args)
 
 static GatherOp gather_for(const fs_model* m, fs_work* w, const float* x, const float* v, float4* vel_dst)
 {
@@ -1110,8 +1106,7 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     fa.damping = prm->boundary_damping;
     fa.control = w->control;
     fa.iterations = io->iterations;
-    hipLaunchKernelGGL(k_dfsph_finalize, dim3(blocks_for(n, kElemBlock)), dim3(kElemBlock), 0, s, fa);
-    FS_LAUNCH_CHECK("k_dfsph_finalize");
+    FS_LAUNCH(k_dfsph_finalize, dim3(blocks_for(n, kElemBlock)), dim3(kElemBlock), 0, s, fa);
     return FS_OK;
 }
 

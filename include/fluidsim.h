@@ -50,6 +50,15 @@ const char* fs_last_error(void);
 /* Library ABI version (major*100 + minor). */
 int32_t fs_version(void);
 
+/* Per-kernel timing for benchmarks (not thread-safe; host-synchronising when
+ * collected).  While enabled, every kernel the library launches is bracketed
+ * by HIP events on its stream.  fs_profile_report writes one line per kernel,
+ * "name launches total_ms", into buf (NUL-terminated, truncated to len) and
+ * returns the full text length; it synchronises the recorded events. */
+void fs_profile_enable(int32_t on);
+void fs_profile_reset(void);
+int32_t fs_profile_report(char* buf, size_t len);
+
 /* ------------------------------------------------------------------------ */
 /* Hash grid                                                                */
 /* ------------------------------------------------------------------------ */

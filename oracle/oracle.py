@@ -427,11 +427,18 @@ def dfsph_step(model: OModel, state_in: OState, cfg: DfsphConfig, dt: float, pre
 
 
 def pbf_w_delta_q(cfg: PbfConfig, precision="f32") -> float:
-    """_w_delta_q  solver_pbf.py:671-674 (host call of the poly6 function)."""
-    lib = _lib(precision)
+    """_w_delta_q  solver_pbf.py:671-674: the poly6 ``@wp.func`` called from
+    Python scope, i.e. evaluated with Python floats (then passed as float32)."""
+    del precision
     h = smoothing_length(cfg)
-    q = float(cfg.scorr_q)
-    return float(lib.ora_poly6_w(lib._real(q * h), lib._real(h)))
+    r = float(cfg.scorr_q) * h
+    if not (h > 0.0 and r < h):
+        return 0.0
+    h2 = h * h
+    rhs = h2 - r * r
+    cube_h = h * h * h
+    k = 315.0 / (64.0 * math.pi * (cube_h * cube_h * cube_h))
+    return k * (rhs * rhs * rhs)
 
 
 def pbf_step(model: OModel, state_in: OState, cfg: PbfConfig, dt: float, precision="f32", round_mode=0):

@@ -1,0 +1,79 @@
+"""Shared inputs for the parity tests: models built through the product's
+Newton-compatible shim and the same arrays as oracle containers."""
+from __future__ import annotations
+
+import numpy as np
+
+from fluid_simulation_amd.examples.dam_break.scene import make_scene
+from fluid_simulation_amd.examples.dam_break.simulation import SOLVERS, material, solver_config
+from fluid_simulation_amd.newton_compat import Axis, ModelBuilder
+
+
+def lattice_block(n_side, spacing, origin=(0.0, 0.0, 0.1), jitter=0.0, seed=0):
+    """Cubic lattice of n_side^3 points (optionally jittered by +-jitter*spacing)."""
+    ax = np.arange(n_side, dtype=np.float64) * spacing
+    g = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3) + np.asarray(origin)
+    if jitter:
+        g = g + np.random.default_rng(seed).uniform(-jitter * spacing, jitter * spacing, g.shape)
+    return g.astype(np.float32)
+
+
+def build_model(method, positions, velocities, *, mass, radius, mat, flags=None, mask=None, gravity=(0, 0, -10.0),
+                device="cuda:0"):
+    n = positions.shape[0]
+    b = ModelBuilder(up_axis=Axis.Z)
+    b.default_particle_radius = radius
+    SOLVERS[method].register_custom_attributes(b)
+    attrs = {f"sph:{k}": (list(v) if np.ndim(v) else [v] * n) for k, v in mat.items()}
+    attrs["sph:particle_mask"] = [0] * n if mask is None else [int(m) for m in mask]
+    b.add_particles(pos=positions, vel=velocities, mass=[mass] * n, radius=[radius] * n, flags=flags,
+                    custom_attributes=attrs)
+    model = b.finalize(device=device)
+    model.set_gravity(gravity)
+    return model
+
+
+def make_solver(method, model, cfg):
+    return SOLVERS[method](model, cfg)
+
+
+def oracle_model(O, model, method):
+    sph = model.sph
+    return O.OModel(mass=model.particle_mass.numpy(), flags=model.particle_flags.numpy(),
+                    mask=sph.particle_mask.numpy(), world=model.particle_world.numpy(),
+                    gravity=model.gravity.numpy(), rest_density=sph.rest_density.numpy(),
+                    surface_tension=sph.surface_tension.numpy(), viscosity=sph.viscosity.numpy(),
+                    stiffness=sph.stiffness.numpy() if method == "wcsph" else None,
+                    exponent=sph.exponent.numpy() if method == "wcsph" else None,
+                    radius=model.particle_radius.numpy())
+
+
+def oracle_config(O, cfg):
+    from dataclasses import asdict
+
+    d = asdict(cfg)
+    if "max_rho_iterations" in d:
+        return O.DfsphConfig(**d)
+    if "lambda_epsilon" in d:
+        return O.PbfConfig(**d)
+    return O.WcsphConfig(**d)
+
+
+def dam_break_inputs(method, target=10000):
+    scene = make_scene(target)
+    cfg = solver_config(method, scene)
+    mat = material(method, scene)
+    return scene, cfg, mat
+
+
+def rel_err(a, b, floor):
+    """max |a-b| / max(|b|, floor), row-norm-wise for (N,3) arrays."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if a.ndim == 2:
+        num = np.linalg.norm(a - b, axis=1)
+        den = np.maximum(np.linalg.norm(b, axis=1), floor)
+    else:
+        num = np.abs(a - b)
+        den = np.maximum(np.abs(b), floor)
+    return float((num / den).max(initial=0.0))

@@ -1,0 +1,112 @@
+"""Hash-grid parity on the GPU: bucket ids, the stable (bucket, index)
+permutation, Warp-layout tables, candidate and within-h neighbour sets must be
+bit-identical to the oracle's."""
+import numpy as np
+import pytest
+import torch
+
+from fluid_simulation_amd.examples.dam_break.scene import make_scene
+from fluid_simulation_amd.hashgrid import HashGrid
+
+pytestmark = pytest.mark.gpu
+
+
+def _points(kind):
+    rng = np.random.default_rng(7)
+    if kind == "random_negative":
+        return rng.uniform(-0.5, 0.5, (1000, 3)).astype(np.float32), 0.05
+    if kind == "strip_aliasing":   # 140 cells long in x: buckets alias (dim 128)
+        x = rng.uniform(0.0, 140 * 0.02, (3000, 1))
+        yz = rng.uniform(-0.03, 0.03, (3000, 2))
+        return np.hstack([x, yz]).astype(np.float32), 0.02
+    if kind == "on_faces":         # points exactly on cell faces and corners
+        h = np.float32(0.25)
+        g = np.arange(-4, 5, dtype=np.float32) * h
+        pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        return np.ascontiguousarray(pts), float(h)
+    if kind == "dam_break_540":
+        s = make_scene(512)
+        return s.positions, 2.0 * s.spacing
+    if kind == "dam_break_10k":
+        s = make_scene(10000)
+        return s.positions, 2.0 * s.spacing
+    if kind == "clustered":
+        c = rng.uniform(0.2, 0.8, (4, 3))
+        pts = (c[rng.integers(0, 4, 4000)] + rng.normal(0, 0.02, (4000, 3))).astype(np.float32)
+        return pts, 0.02
+    raise KeyError(kind)
+
+
+KINDS = ["random_negative", "strip_aliasing", "on_faces", "dam_break_540", "dam_break_10k", "clustered"]
+
+
+def _gpu_grid(pts, h, round_mode=0):
+    g = HashGrid(128, 128, 128, device="cuda:0", round_mode=round_mode)
+    t = torch.from_numpy(pts).cuda()
+    g.reserve(pts.shape[0])
+    g.build(t, h)
+    return g, t
+
+
+@pytest.mark.parametrize("round_mode", [0, 1])
+@pytest.mark.parametrize("kind", KINDS)
+def test_grid_build_bit_exact(kind, round_mode, oracle):
+    pts, h = _points(kind)
+    print(f"grid build {kind} n={len(pts)} h={h} round_mode={round_mode}", flush=True)
+    g, t = _gpu_grid(pts, h, round_mode)
+    og = oracle.OracleGrid(round_mode=round_mode).build(pts, h)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(g.cell_of.cpu().numpy(), og.cell_of)
+    np.testing.assert_array_equal(g.point_ids.cpu().numpy(), og.point_ids)
+    np.testing.assert_array_equal(g.point_cells.cpu().numpy(), og.point_cells)
+    starts, ends = g.cell_tables()
+    np.testing.assert_array_equal(starts.cpu().numpy(), og.cell_starts)
+    np.testing.assert_array_equal(ends.cpu().numpy(), og.cell_ends)
+
+
+@pytest.mark.parametrize("within", [None, "cubic", "pbf"])
+@pytest.mark.parametrize("kind", KINDS)
+def test_candidate_and_neighbor_sets_bit_exact(kind, within, oracle):
+    pts, h = _points(kind)
+    print(f"candidates {kind} within={within}", flush=True)
+    g, t = _gpu_grid(pts, h)
+    off, idx = g.query_lists(t, h, within)
+    og = oracle.OracleGrid().build(pts, h)
+    ooff, oidx = og.candidates(pts, h, within)
+    np.testing.assert_array_equal(off.cpu().numpy(), ooff)
+    np.testing.assert_array_equal(idx.cpu().numpy(), oidx)
+
+
+def test_rebuild_reuses_workspace_and_shrinks(oracle):
+    """Builds with changing point counts on one reserved grid (self-resetting
+    counters and scan state across builds)."""
+    g = HashGrid(128, 128, 128, device="cuda:0")
+    g.reserve(5000)
+    rng = np.random.default_rng(3)
+    for n in (5000, 17, 0, 4096, 1, 2500):
+        pts = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+        g.build(torch.from_numpy(pts).cuda(), 0.07)
+        og = oracle.OracleGrid().build(pts, 0.07)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(g.point_ids.cpu().numpy(), og.point_ids)
+        assert int(g.cell_start[-1].item()) == n
+
+
+def test_full_dam_break_grid_properties_and_parity(oracle):
+    s = make_scene()
+    h = 2.0 * s.spacing
+    g, t = _gpu_grid(s.positions, h)
+    perm = g.point_ids.cpu().numpy()
+    keys = g.point_cells.cpu().numpy()
+    cell_of = g.cell_of.cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(s.count))          # a permutation
+    assert np.all(np.diff(keys) >= 0)                                   # sorted by bucket
+    same = np.diff(keys) == 0
+    assert np.all(np.diff(perm)[same] > 0)                              # stable within bucket
+    np.testing.assert_array_equal(keys, cell_of[perm])
+    og = oracle.OracleGrid().build(s.positions, h)
+    np.testing.assert_array_equal(perm, og.point_ids)
+    counts = g.query_counts(t, h).cpu().numpy()
+    ooff, _ = og.candidates(s.positions, h)
+    np.testing.assert_array_equal(counts, np.diff(ooff))
+    assert counts.min() >= 1

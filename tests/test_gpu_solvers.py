@@ -1,0 +1,154 @@
+"""One-substep parity of the HIP solvers with the FP32 oracle.
+
+Each case feeds the same input state to ``Solver*.step`` (GPU, through the
+C-ABI) and to the oracle's restatement of the reference ``step``; integer
+outputs must be identical and float outputs within 1e-5 relative (row-norm-wise
+for vectors, with a stated floor).  The kernels evaluate the reference's
+float32 expressions in the same order, so most outputs are bit-identical; the
+exact-match fraction is printed.
+"""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+import torch
+
+from fluid_simulation_amd.array import Array
+from fluid_simulation_amd.examples.dam_break.simulation import DamBreakSimulation, solver_config, material
+from fluid_simulation_amd.examples.dam_break.scene import make_scene
+from tests.cases import build_model, lattice_block, make_solver, oracle_config, oracle_model, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1.0e-5
+
+
+def _compare(name, gpu, ref, floor):
+    gpu = np.asarray(gpu)
+    ref = np.asarray(ref)
+    assert gpu.shape == ref.shape, name
+    if ref.dtype.kind == "i":
+        np.testing.assert_array_equal(gpu, ref, err_msg=name)
+        return 1.0
+    exact = float(np.mean(gpu.view(np.uint32) == ref.astype(np.float32).view(np.uint32)))
+    err = rel_err(gpu, ref, floor)
+    print(f"  {name:16s} exact={exact:.4f} rel_err={err:.3e}", flush=True)
+    assert np.all(np.isfinite(gpu) == np.isfinite(ref)), name
+    assert err <= TOL, f"{name}: rel err {err:.3e} > {TOL}"
+    return exact
+
+
+def _compressed_case(method, seed=0, inactive=True):
+    """Jittered lattice at 0.9x spacing (positive pressure), random velocities,
+    straddling two walls, with inactive and masked particles."""
+    d = 0.02
+    pts = lattice_block(12, 0.9 * d, origin=(-0.215, -0.1, -0.004), jitter=0.05, seed=seed)
+    n = pts.shape[0]
+    rng = np.random.default_rng(seed + 1)
+    vel = (rng.normal(size=(n, 3)) * 0.5).astype(np.float32)
+    flags = np.ones(n, np.int32)
+    mask = np.zeros(n, np.int32)
+    if inactive:
+        flags[rng.choice(n, n // 10, replace=False)] = 0
+        mask[rng.choice(n, n // 20, replace=False)] = 1
+    mat = {"rest_density": 1000.0, "surface_tension": 0.0, "viscosity": 1.0e-3}
+    if method == "wcsph":
+        mat.update(stiffness=250000.0, exponent=7.0)
+    common = dict(particle_radius=0.5 * d, particle_length=d, smoothing_length_coff=2.0, boundary_damping=-0.3,
+                  bound_width=0.2, bound_height=0.2, bound_length=0.15)
+    from fluid_simulation_amd.solver.pbf import SolverPBF
+    from fluid_simulation_amd.solver.sph.solver_dfsph import SolverDFSPH
+    from fluid_simulation_amd.solver.sph.solver_wcsph import SolverWCSPH
+    if method == "wcsph":
+        cfg = SolverWCSPH.SphConfig(clamp_negative_pressure=True, **common)
+    elif method == "dfsph":
+        cfg = SolverDFSPH.SphConfig(max_rho_iterations=5, max_vel_iterations=6, warm_start=False, max_kappa=1.0,
+                                    max_kappa_v=100.0, **common)
+    else:
+        cfg = SolverPBF.PbfConfig(max_iterations=4, lambda_epsilon=100.0, min_neighbors_for_lambda=8, scorr_k=0.0,
+                                  max_delta_position=0.25 * d, xsph_c=0.05, **common)
+    model = build_model(method, pts, vel, mass=1000.0 * d ** 3, radius=0.5 * d, mat=mat, flags=flags, mask=mask)
+    return model, cfg
+
+
+def _run_pair(method, model, cfg, dt, oracle, state_in=None):
+    solver = make_solver(method, model, cfg)
+    s_in = model.state() if state_in is None else state_in
+    s_out = model.state()
+    solver.step(s_in, s_out, None, None, dt)
+    torch.cuda.synchronize()
+    om = oracle_model(oracle, model, method)
+    ost = oracle.OState(s_in.particle_q.numpy(), s_in.particle_qd.numpy(),
+                        {k: getattr(s_in.sph, k).numpy() for k in s_in.sph.names()})
+    ocfg = oracle_config(oracle, cfg)
+    fn = {"wcsph": oracle.wcsph_step, "dfsph": oracle.dfsph_step, "pbf": oracle.pbf_step}[method]
+    oout, extras = fn(om, ost, ocfg, dt)
+    return solver, s_out, oout, extras
+
+
+def _check_outputs(method, solver, s_out, oout, extras):
+    print(f"{method}:", flush=True)
+    _compare("x", s_out.particle_q.numpy(), oout.x, 1.0e-2)
+    _compare("v", s_out.particle_qd.numpy(), oout.v, 1.0e-3)
+    _compare("rho", s_out.sph.rho.numpy(), oout.sph["rho"], 1.0)
+    _compare("accel", solver._particle_accel.numpy(), extras["accel"], 1.0)
+    if method == "wcsph":
+        p = oout.sph["pressure"]
+        _compare("pressure", s_out.sph.pressure.numpy(), p, max(1.0, 1e-3 * float(np.abs(p).max(initial=0))))
+    if method == "dfsph":
+        for k in ("factor", "rho_star", "d_rho_dt", "kappa", "kappa_v", "v_star", "neighbor_count"):
+            ref = oout.sph[k]
+            floor = 1e-3 if k == "v_star" else max(1e-12, 1e-3 * float(np.abs(ref).max(initial=0)))
+            _compare(k, getattr(s_out.sph, k).numpy(), ref, floor)
+        it = solver.last_iterations()
+        assert it["density_iterations"] == extras["rho_iterations"]
+        assert it["divergence_iterations"] == extras["vel_iterations"]
+        assert it["density_corrections"] == extras.get("rho_corrections", 0)
+        assert it["divergence_corrections"] == extras.get("vel_corrections", 0)
+    if method == "pbf":
+        for k in ("pbf_lambda", "neighbor_count", "delta_p"):
+            ref = oout.sph[k]
+            floor = max(1e-12, 1e-3 * float(np.abs(ref).max(initial=0)))
+            _compare(k, getattr(s_out.sph, k).numpy(), ref, floor)
+
+
+@pytest.mark.parametrize("method", ["wcsph", "dfsph", "pbf"])
+def test_compressed_block_one_substep(method, oracle):
+    model, cfg = _compressed_case(method)
+    dt = 2.0e-4 if method == "wcsph" else 1.0e-3
+    solver, s_out, oout, extras = _run_pair(method, model, cfg, dt, oracle)
+    _check_outputs(method, solver, s_out, oout, extras)
+
+
+def test_dfsph_warm_start_one_substep(oracle):
+    model, cfg = _compressed_case("dfsph", seed=5)
+    cfg = replace(cfg, warm_start=True)
+    s_in = model.state()
+    rng = np.random.default_rng(11)
+    s_in.sph.kappa.assign(rng.uniform(0, 0.5, model.particle_count).astype(np.float32))
+    s_in.sph.kappa_v.assign(rng.uniform(0, 5.0, model.particle_count).astype(np.float32))
+    solver, s_out, oout, extras = _run_pair("dfsph", model, cfg, 1.0e-3, oracle, state_in=s_in)
+    _check_outputs("dfsph", solver, s_out, oout, extras)
+
+
+@pytest.mark.parametrize("variant", ["rebuild", "scorr", "no_xsph", "zero_iterations"])
+def test_pbf_config_variants(variant, oracle):
+    model, cfg = _compressed_case("pbf", seed=9)
+    cfg = {"rebuild": replace(cfg, rebuild_grid_each_iteration=True),
+           "scorr": replace(cfg, scorr_k=1.0e-4, scorr_n=4.0, scorr_q=0.3, max_delta_position=0.0),
+           "no_xsph": replace(cfg, xsph_c=0.0, boundary_epsilon=1e-4),
+           "zero_iterations": replace(cfg, max_iterations=0)}[variant]
+    solver, s_out, oout, extras = _run_pair("pbf", model, cfg, 1.0e-3, oracle)
+    _check_outputs("pbf", solver, s_out, oout, extras)
+
+
+@pytest.mark.parametrize("method", ["wcsph", "dfsph", "pbf"])
+def test_dam_break_one_substep_after_a_frame(method, oracle):
+    """Advance the 9,936-particle dam break one frame on the GPU, then compare
+    the one-substep map from that state."""
+    scene = make_scene(10000)
+    sim = DamBreakSimulation(scene, method, device="cuda:0")
+    sim.step()
+    dt = sim.frame_dt / 4 if method == "pbf" else min(sim.frame_dt / 4, sim.cfl_limit(*sim._cfl_maxima()))
+    solver, s_out, oout, extras = _run_pair(method, sim.model, sim.solver.config, dt, oracle, state_in=sim.state_0)
+    _check_outputs(method, solver, s_out, oout, extras)
