@@ -182,19 +182,16 @@ class ModelBuilder:
         g[int(self.up_axis)] = float(gravity)
         self.gravity = tuple(g)
         self.default_particle_radius = 0.1
-        self._pos: list = []
-        self._vel: list = []
-        self._mass: list = []
-        self._radius: list = []
-        self._flags: list = []
-        self._world: list = []
+        # per-particle columns kept as lists of numpy chunks
+        self._cols: dict[str, list] = {k: [] for k in ("pos", "vel", "mass", "radius", "flags", "world")}
+        self._count = 0
         self._attrs: dict[str, CustomAttribute] = {}
-        self._attr_values: dict[str, list] = {}
+        self._attr_chunks: dict[str, list] = {}
         self.ground_planes: list = []
 
     @property
     def particle_count(self):
-        return len(self._pos)
+        return self._count
 
     def add_custom_attribute(self, attribute: CustomAttribute) -> None:
         if attribute.key in self._attrs:
@@ -213,23 +210,25 @@ class ModelBuilder:
             np.asarray(radius, dtype=np.float32).reshape(n)
         flags = np.full(n, int(ParticleFlags.ACTIVE), np.int32) if flags is None else \
             np.asarray(flags, dtype=np.int32).reshape(n)
-        start = self.particle_count
+        start = self._count
         for key, values in (custom_attributes or {}).items():
             if key not in self._attrs:
                 raise KeyError(f"custom attribute {key!r} is not registered")
             spec = self._attrs[key]
-            col = self._attr_values.setdefault(key, [])
-            col.extend([spec.default] * (start - len(col)))
-            vals = list(values)
-            if len(vals) != n:
-                raise ValueError(f"custom attribute {key!r}: {len(vals)} values for {n} particles")
-            col.extend(vals)
-        self._pos.extend(pos.tolist())
-        self._vel.extend(vel.tolist())
-        self._mass.extend(mass.tolist())
-        self._radius.extend(radius.tolist())
-        self._flags.extend(flags.tolist())
-        self._world.extend([int(world)] * n)
+            vals = np.asarray(values, dtype=np.int32 if spec.dtype == int32 else np.float32)
+            if spec.dtype == vec3:
+                vals = vals.reshape(-1, 3)
+            if vals.shape[0] != n:
+                raise ValueError(f"custom attribute {key!r}: {vals.shape[0]} values for {n} particles")
+            chunks = self._attr_chunks.setdefault(key, [])
+            have = sum(c.shape[0] for c in chunks)
+            if have < start:
+                chunks.append(_default_block(spec, start - have))
+            chunks.append(vals)
+        for k, v in (("pos", pos), ("vel", vel), ("mass", mass), ("radius", radius), ("flags", flags),
+                     ("world", np.full(n, int(world), np.int32))):
+            self._cols[k].append(v)
+        self._count += n
 
     def add_particle(self, pos, vel, mass, radius=None, flags=None, custom_attributes=None):
         attrs = {k: [v] for k, v in (custom_attributes or {}).items()}
@@ -245,29 +244,42 @@ class ModelBuilder:
         from .hashgrid import HashGrid
 
         m = Model(device)
-        n = self.particle_count
+        n = self._count
         m.particle_count = n
         m.up_axis = self.up_axis
         dev = m.device
-        m.particle_q = Array.from_values(np.asarray(self._pos, np.float32).reshape(n, 3), vec3, dev)
-        m.particle_qd = Array.from_values(np.asarray(self._vel, np.float32).reshape(n, 3), vec3, dev)
-        m.particle_mass = Array.from_values(np.asarray(self._mass, np.float32), float32, dev)
-        m.particle_radius = Array.from_values(np.asarray(self._radius, np.float32), float32, dev)
-        m.particle_flags = Array.from_values(np.asarray(self._flags, np.int32), int32, dev)
-        m.particle_world = Array.from_values(np.asarray(self._world, np.int32), int32, dev)
-        m.particle_max_radius = float(max(self._radius)) if n else 0.0
+
+        def col(k, shape, dt):
+            return np.concatenate(self._cols[k]) if self._cols[k] else np.zeros(shape, dt)
+
+        m.particle_q = Array.from_values(col("pos", (0, 3), np.float32), vec3, dev)
+        m.particle_qd = Array.from_values(col("vel", (0, 3), np.float32), vec3, dev)
+        m.particle_mass = Array.from_values(col("mass", (0,), np.float32), float32, dev)
+        radius = col("radius", (0,), np.float32)
+        m.particle_radius = Array.from_values(radius, float32, dev)
+        m.particle_flags = Array.from_values(col("flags", (0,), np.int32), int32, dev)
+        m.particle_world = Array.from_values(col("world", (0,), np.int32), int32, dev)
+        m.particle_max_radius = float(radius.max()) if n else 0.0
         m.gravity = Array.from_values(np.asarray([self.gravity], np.float32), vec3, dev)
         m.num_worlds = 1
         m._attr_specs = list(self._attrs.values())
         for spec in m._attr_specs:
-            col = self._attr_values.get(spec.key)
-            if col is not None:
-                col = col + [spec.default] * (n - len(col))
-                m._attr_values[spec.key] = Array.from_values(col, spec.dtype, dev, count=n)
+            chunks = self._attr_chunks.get(spec.key)
+            if chunks is not None:
+                have = sum(c.shape[0] for c in chunks)
+                if have < n:
+                    chunks = chunks + [_default_block(spec, n - have)]
+                m._attr_values[spec.key] = Array.from_values(np.concatenate(chunks), spec.dtype, dev, count=n)
             if spec.assignment == AttributeAssignment.MODEL:
                 _set_namespaced(m, spec, m._attribute_array(spec, n))
         m.particle_grid = HashGrid(128, 128, 128, device=dev) if n > 1 else None
         return m
+
+
+def _default_block(spec, count):
+    if spec.dtype == vec3:
+        return np.tile(np.asarray(spec.default, np.float32).reshape(1, 3), (count, 1))
+    return np.full(count, spec.default, np.int32 if spec.dtype == int32 else np.float32)
 
 
 class SolverBase:
