@@ -21,6 +21,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -56,6 +57,9 @@ def parse():
     ap.add_argument("--solvers", default="wcsph,dfsph,pbf")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-substeps", type=int, default=40)
+    ap.add_argument("--copy-calibration", action="store_true", help="run only the PMC copy calibration kernel")
+    ap.add_argument("--micro", default="", help="comma-separated particle counts for the grid/density/force "
+                                                "microbenchmark (e.g. 4000000,16000000)")
     return ap.parse_args()
 
 
@@ -89,7 +93,11 @@ def bench_solver(method, scene, steps, warmup):
     for _ in range(steps):
         sim._substep(dt)
     lib.fs_profile_enable(0)
-    prof = _lib.profile_report()
+    prof = {}
+    for name, (cnt, ms) in _lib.profile_report().items():
+        key = re.sub(r"<.*>", "", name.strip("()"))
+        c0, m0 = prof.get(key, (0, 0.0))
+        prof[key] = (c0 + cnt, m0 + ms)
     sim.check_finite()
     n = scene.count
     rate = n * steps / elapsed
@@ -135,8 +143,69 @@ def cpu_baseline(scene, substeps):
                       f"{threads} threads, {el:.1f} s"}
 
 
+def copy_calibration():
+    """Plain device copy of 256 MiB, three times: the reference kernel for
+    scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+
+    a = torch.ones(64 * 1024 * 1024, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": 2 * a.numel() * 4}), flush=True)
+
+
+def micro(sizes, reps):
+    """Grid build, density pass and force pass at 4M / 16M points, uniform and
+    clustered (per-kernel HIP-event timing, mean over ``reps`` substeps)."""
+    import torch
+
+    from fluid_simulation_amd import _lib
+    from fluid_simulation_amd.examples import microbench as mb
+    from fluid_simulation_amd.solver.sph.solver_wcsph import SolverWCSPH
+
+    lib = _lib.load()
+    for n in sizes:
+        for dist in ("uniform", "clustered"):
+            pts = mb.uniform_points(n) if dist == "uniform" else mb.clustered_points(n)
+            model, cfg = mb.wcsph_model(pts)
+            solver = SolverWCSPH(model, cfg)
+            s0, s1 = model.state(), model.state()
+            solver.step(s0, s1, None, None, 1e-5)          # warm-up
+            torch.cuda.synchronize()
+            lib.fs_profile_reset()
+            lib.fs_profile_enable(1)
+            for _ in range(reps):
+                solver.step(s0, s1, None, None, 1e-5)
+            lib.fs_profile_enable(0)
+            prof = {}
+            for name, (cnt, ms) in _lib.profile_report().items():
+                key = re.sub(r"<.*>", "", name.strip("()"))
+                c0, m0 = prof.get(key, (0, 0.0))
+                prof[key] = (c0 + cnt, m0 + ms)
+            build = sum(prof[k][1] for k in ("k_hash_count", "k_bucket_scan", "k_scatter", "k_fix_gather")) / reps
+            dens = prof["k_wcsph_density"][1] / reps
+            force = prof["k_wcsph_force"][1] / reps
+            counts = model.particle_grid.query_counts(s0.particle_q, solver.smoothing_length, "cubic")
+            nbr = float(counts.float().mean().item()) - 1.0
+            print(json.dumps({"micro": dist, "particles": n, "h": solver.smoothing_length, "mean_neighbors": nbr,
+                              "grid_build_ms": build, "density_pass_ms": dens, "force_pass_ms": force,
+                              "density_GBs_alg": PASS_BYTES["k_wcsph_density"] * n / dens / 1e6,
+                              "force_GBs_alg": PASS_BYTES["k_wcsph_force"] * n / force / 1e6}), flush=True)
+            del solver, model, s0, s1
+            torch.cuda.empty_cache()
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.micro:
+        micro([int(x) for x in args.micro.split(",")], args.steps)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path does not shard "
               "across GPUs here)", file=sys.stderr)

@@ -38,10 +38,10 @@ class FsModel(ctypes.Structure):
 
 class FsWork(ctypes.Structure):
     _fields_ = [
-        ("capacity", c_int32),
+        ("capacity", c_int32), ("list_capacity", c_int32),
         ("pos", c_void_p), ("pos_alt", c_void_p), ("vel", c_void_p), ("sca", c_void_p), ("wvol", c_void_p),
-        ("flags", c_void_p), ("x_pred", c_void_p), ("v_pred", c_void_p), ("partials", c_void_p),
-        ("control", c_void_p), ("sync", c_void_p),
+        ("flags", c_void_p), ("x_pred", c_void_p), ("v_pred", c_void_p), ("nbr", c_void_p),
+        ("nbr_count", c_void_p), ("partials", c_void_p), ("control", c_void_p), ("sync", c_void_p),
     ]
 
 
@@ -111,8 +111,8 @@ SIGNATURES = {
     "fs_grid_count_candidates": (c_int32, [POINTER(FsGrid), c_void_p, c_float, c_int32, c_void_p, c_void_p]),
     "fs_grid_list_candidates": (c_int32, [POINTER(FsGrid), c_void_p, c_float, c_int32, c_void_p, c_void_p,
                                           c_void_p]),
-    "fs_work_bytes": (c_size_t, [c_int32]),
-    "fs_work_init": (c_int32, [POINTER(FsWork), c_void_p, c_size_t, c_int32, c_void_p]),
+    "fs_work_bytes": (c_size_t, [c_int32, c_int32]),
+    "fs_work_init": (c_int32, [POINTER(FsWork), c_void_p, c_size_t, c_int32, c_int32, c_void_p]),
     "fs_wcsph_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsWcsphParams),
                                 POINTER(FsWcsphIO), c_void_p]),
     "fs_dfsph_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsDfsphParams),

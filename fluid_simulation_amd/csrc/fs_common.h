@@ -83,11 +83,11 @@ __device__ __forceinline__ float cubic_w(float r_norm, float h)
     return res;
 }
 
-// get_cubic_derivative  solver_wcsph.py:36-50
-__device__ __forceinline__ v3 cubic_grad(v3 r, float h)
+// get_cubic_derivative  solver_wcsph.py:36-50, with r_norm = wp.length(r)
+// supplied by the caller (it is the same float value every time)
+__device__ __forceinline__ v3 cubic_grad_r(v3 r, float r_norm, float h)
 {
     v3 res = V(0.0f, 0.0f, 0.0f);
-    float r_norm = vlen(r);
     float k = 8.0f / (kPi * (h * h * h));
     float q = r_norm / h;
     if (r_norm > 1.0e-5f && q <= 1.0f) {
@@ -101,6 +101,7 @@ __device__ __forceinline__ v3 cubic_grad(v3 r, float h)
     }
     return res;
 }
+__device__ __forceinline__ v3 cubic_grad(v3 r, float h) { return cubic_grad_r(r, vlen(r), h); }
 
 // PBF get_cubic = poly6  solver_pbf.py:23-32
 __device__ __forceinline__ float poly6_w(float r_norm, float h)
@@ -117,10 +118,9 @@ __device__ __forceinline__ float poly6_w(float r_norm, float h)
 }
 
 // PBF get_cubic_derivative = spiky gradient  solver_pbf.py:35-45
-__device__ __forceinline__ v3 spiky_grad(v3 r, float h)
+__device__ __forceinline__ v3 spiky_grad_r(v3 r, float r_norm, float h)
 {
     v3 res = V(0.0f, 0.0f, 0.0f);
-    float r_norm = vlen(r);
     if (h > 0.0f && r_norm > 1.0e-6f && r_norm < h) {
         float h3 = h * h * h;
         float h6 = h3 * h3;
@@ -130,6 +130,7 @@ __device__ __forceinline__ v3 spiky_grad(v3 r, float h)
     }
     return res;
 }
+__device__ __forceinline__ v3 spiky_grad(v3 r, float h) { return spiky_grad_r(r, vlen(r), h); }
 
 // cal_acc_with_non_pressure  solver_wcsph.py:63-89 (= solver_dfsph.py:67-93)
 __device__ __forceinline__ v3 acc_non_pressure(v3 a, float rho_nei, v3 vel, v3 vel_nei, float mass, float mass_nei,
@@ -280,43 +281,111 @@ struct QueryRows {
     __device__ __forceinline__ bool valid() const { return nz > 0; }
 };
 
+// Per-particle neighbour list: the within-cut candidates of a query in visit
+// order, entry e of sorted slot s at idx[e * n + s] (coalesced across lanes).
+// cnt[s] > cap marks an overflowed list; its users fall back to scanning.
+struct NbrList {
+    int* idx;
+    int* cnt;
+    int cap;
+    int n;
+    int mode;      // kListNone: scan only; kListBuild: scan and record; kListUse: replay
+    float cut2;    // squared record radius (>= the pass cut; larger for a skin)
+};
+constexpr int kListNone = 0;
+constexpr int kListBuild = 1;
+constexpr int kListUse = 2;
+constexpr int kBatch = 8;
+
 // Visit the candidates of hash_grid_query(p, r) in Warp order, but call
 // `body(k)` (k = sorted slot) only for those with squared distance not above
 // `cut2` -- every skipped candidate has q > 1 and contributes exactly zero to
-// every kernel sum here.  The survivors are compacted through a per-lane LDS
-// list of CAP entries so that the expensive body runs on full waves.
-// `count(k)` is called for every candidate (used for DFSPH's unfiltered
-// neighbour count); pass a no-op when not needed.
+// every kernel sum here.  Candidate positions are loaded kBatch at a time;
+// survivors are compacted through a per-lane LDS list of CAP entries so that
+// the expensive body runs on full waves.  `count(k)` is called for every
+// candidate (DFSPH's unfiltered neighbour count).  With a kListBuild list the
+// survivors (radius^2 <= L.cut2) are also recorded for later passes.
 template <int BLOCK, int CAP, class Count, class Body>
-__device__ __forceinline__ void for_each_neighbor(const GridView& g, const float4* __restrict__ pos, float px,
-                                                  float py, float pz, float r, float cut2, int* __restrict__ lds,
-                                                  Count&& count, Body&& body)
+__device__ __forceinline__ void scan_neighbors(const GridView& g, const float4* __restrict__ pos, int s, float px,
+                                               float py, float pz, float r, float cut2, int* __restrict__ lds,
+                                               const NbrList& L, Count&& count, Body&& body)
 {
+    static_assert(CAP >= 2 * kBatch, "list capacity");
     QueryRows it;
     it.begin(g, px, py, pz, r);
-    if (!it.valid()) return;
-    int* list = lds + threadIdx.x;
-    bool done = false;
-    while (true) {
-        int cnt = 0;
-        while (cnt < CAP) {
-            if (it.k >= it.kend) {
-                if (!it.advance(g)) {
-                    done = true;
-                    break;
+    const bool record = L.mode == kListBuild;
+    const float keep2 = record ? fmax_(L.cut2, cut2) : cut2;
+    int written = 0;
+    if (it.valid()) {
+        int* list = lds + threadIdx.x;
+        bool done = false;
+        while (true) {
+            int cnt = 0;
+            while (!done && cnt <= CAP - kBatch) {
+                const int k0 = it.k;
+                const int rem = it.kend - k0;
+                if (rem <= 0) {
+                    if (!it.advance(g)) done = true;
+                    continue;
                 }
-                continue;
+                const int take = rem < kBatch ? rem : kBatch;
+                float4 q[kBatch];
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) q[u] = pos[k0 + (u < take ? u : take - 1)];
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) {
+                    if (u < take) {
+                        count(k0 + u);
+                        float ddx = px - q[u].x, ddy = py - q[u].y, ddz = pz - q[u].z;
+                        float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        if (!(d2 > keep2)) list[(cnt++) * BLOCK] = k0 + u;
+                    }
+                }
+                it.k = k0 + take;
             }
-            int k = it.k++;
-            float4 q = pos[k];
-            count(k);
-            float ddx = px - q.x, ddy = py - q.y, ddz = pz - q.z;
-            float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (!(d2 > cut2)) list[(cnt++) * BLOCK] = k;
+            for (int e = 0; e < cnt; ++e) {
+                const int k = list[e * BLOCK];
+                if (record) {
+                    if (written < L.cap) L.idx[written * L.n + s] = k;
+                    ++written;
+                }
+                body(k);
+            }
+            if (done) break;
         }
-        for (int e = 0; e < cnt; ++e) body(list[e * BLOCK]);
-        if (done) break;
     }
+    if (record) L.cnt[s] = written <= L.cap ? written : L.cap + 1;
+}
+
+// Neighbour loop of one pass: replay the recorded list when there is one (and
+// it did not overflow), otherwise scan the grid.
+template <int BLOCK, int CAP, class Count, class Body>
+__device__ __forceinline__ void for_each_neighbor(const GridView& g, const float4* __restrict__ pos, int s, float px,
+                                                  float py, float pz, float r, float cut2, int* __restrict__ lds,
+                                                  const NbrList& L, Count&& count, Body&& body)
+{
+    if (L.mode == kListUse) {
+        const int m = L.cnt[s];
+        if (m <= L.cap) {
+            const int* __restrict__ idx = L.idx + s;
+            int e = 0;
+            for (; e + 4 <= m; e += 4) {
+                const int k0 = idx[(e + 0) * L.n], k1 = idx[(e + 1) * L.n];
+                const int k2 = idx[(e + 2) * L.n], k3 = idx[(e + 3) * L.n];
+                body(k0);
+                body(k1);
+                body(k2);
+                body(k3);
+            }
+            for (; e < m; ++e) body(idx[e * L.n]);
+            return;
+        }
+        NbrList none = L;
+        none.mode = kListNone;
+        scan_neighbors<BLOCK, CAP>(g, pos, s, px, py, pz, r, cut2, lds, none, count, body);
+        return;
+    }
+    scan_neighbors<BLOCK, CAP>(g, pos, s, px, py, pz, r, cut2, lds, L, count, body);
 }
 
 // Block index -> tile index so that each of the 8 XCDs (which receive

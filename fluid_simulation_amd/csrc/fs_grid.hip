@@ -8,8 +8,8 @@
 //   4. k_fix_gather   order each bucket's members by original index (the
 //                     stable (bucket, index) permutation of Warp's build),
 //                     and gather particle data into that sorted order.
-// The scan's tile counter and descriptors are self-resetting (epoch tagged),
-// so a build can be replayed from a captured graph.
+// The scan's look-back descriptors are re-zeroed by k_scatter, so every build
+// (also one replayed from a captured graph) starts from a clean state.
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -115,17 +115,17 @@ struct HashArgs {
     GridView g;
     PreOp pre;
     int n;
+    const int* order;   // processing order (the previous build's sorted order) or null
     int* cell_of;
     int* rank;
     int* count;
 };
 
+// Position of point i after the solver's pre-op; writes the pre-op outputs
+// and the bucket id, returns the bucket.
 template <int MODE>
-__global__ __launch_bounds__(kElemBlock) void k_hash_count(HashArgs a)
+__device__ __forceinline__ int hash_point(const HashArgs& a, int i)
 {
-    const int i = blockIdx.x * kElemBlock + threadIdx.x;
-    if (a.pre.control_reset && i < 16) a.pre.control_reset[i] = 0;
-    if (i >= a.n) return;
     const PreOp& p = a.pre;
     v3 x = ld3(p.x_in, i);
     if (MODE == kPreIntegrate) {
@@ -158,34 +158,61 @@ __global__ __launch_bounds__(kElemBlock) void k_hash_count(HashArgs a)
         st3(p.v_out, i, v);
         st3(p.x_out, i, x);
     }
-    int c = bucket_of(a.g, x.x, x.y, x.z);
+    const int c = bucket_of(a.g, x.x, x.y, x.z);
     a.cell_of[i] = c;
-    a.rank[i] = atomicAdd(&a.count[c], 1);
+    return c;
 }
 
-// Descriptor: [63:62] status (1 aggregate, 2 inclusive prefix),
-//             [61:32] epoch, [31:0] value.
-__device__ __forceinline__ uint64_t scan_desc(uint32_t status, uint32_t epoch, int value)
+// Bucket every point and take a slot rank from its bucket counter.  Lanes are
+// fed in the previous build's sorted order when there is one, so a wave holds
+// few distinct buckets; the counter increments are aggregated per bucket and
+// wave (one atomic per distinct bucket).  Ranks only need to be distinct:
+// k_fix_gather orders each bucket by original index afterwards.
+template <int MODE>
+__global__ __launch_bounds__(kElemBlock) void k_hash_count(HashArgs a)
 {
-    return ((uint64_t)status << 62) | ((uint64_t)(epoch & 0x3FFFFFFFu) << 32) | (uint64_t)(uint32_t)value;
+    const int t = blockIdx.x * kElemBlock + threadIdx.x;
+    if (a.pre.control_reset && t < 16) a.pre.control_reset[t] = 0;
+    const bool valid = t < a.n;
+    const int i = valid ? (a.order ? a.order[t] : t) : 0;
+    int c = -1;
+    if (valid) c = hash_point<MODE>(a, i);
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(valid);
+    int rank = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int b = __shfl(c, leader, 64);
+        const unsigned long long same = __ballot(valid && c == b) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&a.count[b], __popcll(same));
+        base = __shfl(base, leader, 64);
+        if ((same >> lane) & 1ull) rank = base + __popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    if (valid) a.rank[i] = rank;
 }
 
+
+// Descriptor of one scan tile: [63:62] status (0 not ready, 1 aggregate,
+// 2 inclusive prefix), [31:0] value.  Flag and value travel in one 64-bit
+// word, so relaxed agent-scope atomics suffice.  k_scatter re-zeroes the
+// descriptors after every scan, so the next build starts clean.
+__device__ __forceinline__ uint64_t scan_desc(uint32_t status, int value)
+{
+    return ((uint64_t)status << 62) | (uint64_t)(uint32_t)value;
+}
+
+// Single-pass exclusive scan with decoupled look-back.  Tile = blockIdx.x:
+// workgroups are dispatched in index order, so every predecessor of a
+// resident tile has started and will publish.
 __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restrict__ in, int* __restrict__ out,
-                                                            int m, int ntiles, uint64_t* desc, uint32_t* sync)
+                                                            int m, int ntiles, uint64_t* desc)
 {
-    __shared__ int s_tile;
-    __shared__ uint32_t s_epoch;
     __shared__ int s_wsum[kScanBlock / 64];
     __shared__ int s_excl;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-    if (t == 0) {
-        s_tile = (int)atomicAdd(&sync[0], 1u);
-        s_epoch = __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0x3FFFFFFFu;
-    }
-    __syncthreads();
-    const int tile = s_tile;
-    const uint32_t epoch = s_epoch;
-    if (tile >= ntiles) return;   // only reachable with corrupted counters
+    const int tile = blockIdx.x;
     const int base = tile * kScanTile + t * kScanItems;
 
     int v[kScanItems];
@@ -226,28 +253,22 @@ __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restric
     if (wid == 0) {
         int excl = 0;
         if (tile == 0) {
-            if (lane == 0)
-                __hip_atomic_store(&desc[0], scan_desc(2, epoch, agg), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) __hip_atomic_store(&desc[0], scan_desc(2, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             if (lane == 0)
-                __hip_atomic_store(&desc[tile], scan_desc(1, epoch, agg), __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&desc[tile], scan_desc(1, agg), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int look = tile - 1;
             while (true) {
-                int idx = look - lane;
+                const int idx = look - lane;
                 int status = 2, val = 0;
                 if (idx >= 0) {
-                    uint64_t d = __hip_atomic_load(&desc[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                    uint32_t e = (uint32_t)(d >> 32) & 0x3FFFFFFFu;
-                    status = (e == epoch) ? (int)(d >> 62) : 0;
+                    uint64_t d = __hip_atomic_load(&desc[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    status = (int)(d >> 62);
                     val = (int)(uint32_t)d;
                 }
-                if (__any(status == 0)) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
+                if (__any(status == 0)) continue;
                 unsigned long long pm = __ballot(status == 2);
-                int first = pm ? (__ffsll(pm) - 1) : 64;
+                const int first = pm ? (__ffsll(pm) - 1) : 64;
                 int contrib = (lane <= first) ? val : 0;
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off, 64);
@@ -256,7 +277,7 @@ __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restric
                 look -= 64;
             }
             if (lane == 0)
-                __hip_atomic_store(&desc[tile], scan_desc(2, epoch, excl + agg), __ATOMIC_RELEASE,
+                __hip_atomic_store(&desc[tile], scan_desc(2, excl + agg), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) s_excl = excl;
@@ -280,21 +301,15 @@ __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restric
             if (base + j < m) out[base + j] = excl + (j == 0 ? 0 : v[j - 1]);
     }
     if (tile == ntiles - 1 && t == 0) out[m] = s_excl + agg;
-    if (t == 0) {
-        __threadfence();
-        if (atomicAdd(&sync[1], 1u) == (uint32_t)(ntiles - 1)) {
-            __hip_atomic_store(&sync[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&sync[2], (epoch + 1u) & 0x3FFFFFFFu, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
 }
 
 __global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __restrict__ cell_of,
                                                         const int* __restrict__ rank, const int* __restrict__ start,
-                                                        int* __restrict__ scatter_ids, int* __restrict__ count)
+                                                        int* __restrict__ scatter_ids, int* __restrict__ count,
+                                                        uint64_t* scan_desc_clear, int ntiles)
 {
     const int i = blockIdx.x * kElemBlock + threadIdx.x;
+    if (i < ntiles) scan_desc_clear[i] = 0;
     if (i >= n) return;
     int c = cell_of[i];
     int slot = start[c] + rank[i];
@@ -399,12 +414,15 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
         set_error("grid build: radius must be positive");
         return FS_ERR_ARG;
     }
+    const bool reorder = g->num_points == n && g->cell_width > 0.0f;
     g->num_points = n;
     g->cell_width = radius;
     g->cell_width_inv = 1.0f / radius;
     const GridView gv = grid_view(g);
     const int m = g->dim_x * g->dim_y * g->dim_z;
-    HashArgs ha{gv, pre, n, g->cell_of, g->rank, g->cell_count};
+    // the previous build's sorted order, when it covered the same points
+    const int* order = reorder ? g->point_ids : nullptr;
+    HashArgs ha{gv, pre, n, order, g->cell_of, g->rank, g->cell_count};
     const int nb = blocks_for(n, kElemBlock);
     if (nb > 0 || pre.control_reset) {
         const int nbh = nb > 0 ? nb : 1;
@@ -416,10 +434,11 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
     }
     const int ntiles = scan_tiles(g);
     FS_LAUNCH(k_bucket_scan, dim3(ntiles), dim3(kScanBlock), 0, s, g->cell_count, g->cell_start, m, ntiles,
-              g->scan_state, g->sync);
+              g->scan_state);
+    const int nbs = nb > blocks_for(ntiles, kElemBlock) ? nb : blocks_for(ntiles, kElemBlock);
+    FS_LAUNCH(k_scatter, dim3(nbs), dim3(kElemBlock), 0, s, n, g->cell_of, g->rank, g->cell_start, g->scatter_ids,
+              g->cell_count, g->scan_state, ntiles);
     if (nb == 0) return FS_OK;
-    FS_LAUNCH(k_scatter, dim3(nb), dim3(kElemBlock), 0, s, n, g->cell_of, g->rank, g->cell_start, g->scatter_ids,
-              g->cell_count);
     FS_LAUNCH(k_fix_gather, dim3(nb), dim3(kElemBlock), 0, s, n, g->scatter_ids, g->cell_of, g->cell_start,
               g->point_ids, g->point_cells, ga);
     return FS_OK;

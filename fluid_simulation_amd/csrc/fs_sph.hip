@@ -9,15 +9,22 @@
 // also into sorted scratch for the next pass.  Sums run over candidates in
 // hash_grid_query visit order with the reference's float32 expressions, so a
 // pass reproduces the reference kernel's arithmetic operation for operation.
+//
+// A pass that scans the grid records each particle's within-cut candidates
+// (visit order) in a neighbour list; the next pass over the same positions
+// and the same grid replays it instead of scanning the 27 buckets again.  A
+// particle's candidates depend on its current position (the query range is
+// recomputed from it), so every pass after the positions change scans again.
 #include "fs_internal.h"
 
 namespace fs {
 
 constexpr int kPassBlock = 256;
-constexpr int kListCap = 32;
+constexpr int kScanCap = 32;    // LDS survivor list of a scanning pass
+constexpr int kReplayCap = 16;  // ... of a list-replaying pass (only its overflow fallback scans)
 constexpr float kCutSlack = 1.0001f;   // candidates with d2 > h^2*slack have q > 1
 
-// loop-control words in fs_work::control
+// loop-control words in fs_work::control (int32[64])
 enum Control : int {
     kStopRho = 0,
     kStopVel = 1,
@@ -25,7 +32,7 @@ enum Control : int {
     kCorrRho = 3,
     kItVel = 4,
     kCorrVel = 5,
-    kTicket = 8,
+    kControlWords = 64,
 };
 
 struct PassCommon {
@@ -36,6 +43,7 @@ struct PassCommon {
     const int* __restrict__ sflags;
     float h;
     float cut2;
+    NbrList L;
 };
 
 __device__ __forceinline__ int pass_slot() { return xcd_tile() * kPassBlock + threadIdx.x; }
@@ -44,15 +52,18 @@ struct NoCount {
     __device__ __forceinline__ void operator()(int) const {}
 };
 
+#define NEIGHBORS(c, s, px, py, pz, count, body)                                                           \
+    for_each_neighbor<kPassBlock, CAP>((c).g, (c).pos, (s), (px), (py), (pz), (c).h, (c).cut2, lds, (c).L, \
+                                            count, body)
+
 // ---------------------------------------------------------------------------
-// block reduction of the DFSPH residual; the last block decides the early exit
+// DFSPH residual: each block of the rate pass stores its partial sums; a
+// one-block kernel then reduces them in a fixed order and decides the early exit
 // ---------------------------------------------------------------------------
-__device__ void residual_reduce(double v, double c, double* partials, int* control, int stop_idx, int it_idx,
-                                int iter, double eta)
+__device__ void residual_partials(double v, double c, double* partials)
 {
     __shared__ double s_v[kPassBlock / 64];
     __shared__ double s_c[kPassBlock / 64];
-    __shared__ int s_last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -64,33 +75,37 @@ __device__ void residual_reduce(double v, double c, double* partials, int* contr
         s_c[wid] = c;
     }
     __syncthreads();
-    const int nb = gridDim.x;
     if (threadIdx.x == 0) {
         double bv = 0.0, bc = 0.0;
         for (int w = 0; w < kPassBlock / 64; ++w) {
             bv += s_v[w];
             bc += s_c[w];
         }
-        __hip_atomic_store(&partials[2 * blockIdx.x], bv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[2 * blockIdx.x + 1], bc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        int t = atomicAdd(&control[kTicket], 1);
-        s_last = (t == nb - 1);
+        partials[2 * blockIdx.x] = bv;
+        partials[2 * blockIdx.x + 1] = bc;
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+}
+
+// residual = mean over dynamic particles of max(x, 0); stop when
+// iter > 0 and residual <= eta  (solver_dfsph.py:579-596, 733-735, 849-851)
+__global__ __launch_bounds__(kPassBlock) void k_dfsph_residual(const double* __restrict__ partials, int nb,
+                                                               int* control, int stop_idx, int it_idx, int iter,
+                                                               double eta)
+{
+    __shared__ double s_v[kPassBlock / 64];
+    __shared__ double s_c[kPassBlock / 64];
+    if (control[stop_idx]) return;
     double tv = 0.0, tc = 0.0;
     for (int b = threadIdx.x; b < nb; b += kPassBlock) {
-        tv += __hip_atomic_load(&partials[2 * b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tc += __hip_atomic_load(&partials[2 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        tv += partials[2 * b];
+        tc += partials[2 * b + 1];
     }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         tv += __shfl_xor(tv, off, 64);
         tc += __shfl_xor(tc, off, 64);
     }
-    __syncthreads();
     if (lane == 0) {
         s_v[wid] = tv;
         s_c[wid] = tc;
@@ -102,10 +117,9 @@ __device__ void residual_reduce(double v, double c, double* partials, int* contr
             sv += s_v[w];
             sc += s_c[w];
         }
-        double residual = sc > 0.0 ? sv / sc : 0.0;
+        const double residual = sc > 0.0 ? sv / sc : 0.0;
         control[it_idx] += 1;
         if (iter > 0 && residual <= eta) control[stop_idx] = 1;
-        control[kTicket] = 0;
     }
 }
 
@@ -122,38 +136,42 @@ struct WcsphDensityArgs {
     int clamp;
     float* rho;
     float* pressure;
-    float4* sca;   // rho, p
+    float4* sca;    // rho, p, p/max(rho^2,1e-6), -
+    float4* vel;    // .w <- m/max(rho,1e-6)
 };
 
 // compute_density_pressure  solver_wcsph.py:119-158
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_wcsph_density(WcsphDensityArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
     const int s = pass_slot();
     if (s >= a.c.n) return;
     const int i = a.c.perm[s];
+    const float4 xi = a.c.pos[s];
     if (!(a.c.sflags[s] & kActive)) {
         a.rho[i] = 0.0f;
         a.pressure[i] = 0.0f;
         a.sca[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.vel[s].w = xi.w / fmax_(0.0f, 1.0e-6f);
+        if (a.c.L.mode == kListBuild) a.c.L.cnt[s] = 0;
         return;
     }
-    const float4 xi = a.c.pos[s];
     const float h = a.c.h;
     float rho_temp = 0.0f;
-    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
-                                            [&](int k) {
-                                                if (!(a.c.sflags[k] & kActive)) return;
-                                                v3 dist = vsub(xyz(xi), xyz(a.c.pos[k]));
-                                                float mass_nei = a.wvol[k];
-                                                rho_temp += mass_nei * cubic_w(vlen(dist), h);
-                                            });
+    NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+        if (!(a.c.sflags[k] & kActive)) return;
+        v3 dist = vsub(xyz(xi), xyz(a.c.pos[k]));
+        float mass_nei = a.wvol[k];
+        rho_temp += mass_nei * cubic_w(vlen(dist), h);
+    });
     const float rho_i = rho_temp;
     float p = a.stiffness[i] * (fpow_(rho_i / fmax_(a.rest_density[i], 1.0e-6f), a.exponent[i]) - 1.0f);
     if (a.clamp) p = fmax_(p, 0.0f);
     a.rho[i] = rho_i;
     a.pressure[i] = p;
-    a.sca[s] = make_float4(rho_i, p, 0.0f, 0.0f);
+    a.sca[s] = make_float4(rho_i, p, p / fmax_(rho_i * rho_i, 1.0e-6f), 0.0f);
+    a.vel[s].w = xi.w / fmax_(rho_i, 1.0e-6f);
 }
 
 struct WcsphForceArgs {
@@ -176,10 +194,16 @@ struct WcsphForceArgs {
 };
 
 // acceleration_wcsph + integrate_velocity + integrate_positions + apply_bounds
-// solver_wcsph.py:161-305
+// solver_wcsph.py:161-305.  Per neighbour the operations of
+// cal_acc_with_non_pressure and cal_acc_with_pressure (:63-106) are evaluated
+// with the kernel gradient computed once and the per-particle quotients
+// p/max(rho^2,1e-6), m/max(rho,1e-6) precomputed -- the same float operations
+// on the same operands.  A zero cohesion (gamma) or viscosity (mu) coefficient
+// contributes an exact zero, so its term is skipped.
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_wcsph_force(WcsphForceArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
     const int s = pass_slot();
     if (s >= a.c.n) return;
     const int i = a.c.perm[s];
@@ -193,37 +217,43 @@ __global__ __launch_bounds__(kPassBlock) void k_wcsph_force(WcsphForceArgs a)
     const v3 xi = xyz(xi4);
     const v3 vi = xyz(a.vel[s]);
     const float4 si = a.sca[s];
-    const float rho_i = si.x, p_i = si.y, m_i = xi4.w;
+    const float dp_i = si.z, m_i = xi4.w;
     const float mu = a.viscosity[i], gamma = a.surface_tension[i];
     const float h = a.c.h, pr = a.particle_radius;
+    const float visc_den0 = 0.01f * (h * h);
+    const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
-    for_each_neighbor<kPassBlock, kListCap>(
-        a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{}, [&](int k) {
-            if (k == s || !(a.c.sflags[k] & kActive)) return;
-            const float4 xj = a.c.pos[k];
-            const float4 sj = a.sca[k];
-            v3 direction = vsub(xi, xyz(xj));
-            float distance = vlen(direction);
-            float mass_nei = xj.w;
-            float rho_nei = sj.x;
-            float p_nei = sj.y;
-            acc = acc_non_pressure(acc, rho_nei, vi, xyz(a.vel[k]), m_i, mass_nei, mu, gamma, direction,
-                                   fmax_(distance, pr), distance, h);
-            acc = acc_pressure(acc, direction, p_i, p_nei, rho_i, rho_nei, mass_nei, h);
-        });
+    NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+        if (k == s || !(a.c.sflags[k] & kActive)) return;
+        const float4 xj = a.c.pos[k];
+        const float4 sj = a.sca[k];
+        const v3 d = vsub(xi, xyz(xj));
+        const float distance = vlen(d);
+        const float mass_nei = xj.w;
+        const v3 grad = cubic_grad_r(d, distance, h);
+        if (gamma != 0.0f)
+            acc = vsub(acc, smul(gamma / m_i * mass_nei, muls(d, cubic_w(fmax_(distance, pr), h))));
+        if (mu != 0.0f) {
+            const float4 vj = a.vel[k];
+            float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
+            float sv = mu10 * vj.w * v_current_nei / (distance * distance + visc_den0);
+            acc = vadd(acc, smul(sv, grad));
+        }
+        acc = vadd(acc, smul(-mass_nei * (dp_i + sj.z), grad));
+    });
     int w = a.world[i];
     w = w > 0 ? w : 0;
     const v3 acc_g = vadd(acc, ld3(a.gravity, w));
     st3(a.accel, i, acc_g);
     v3 v = vadd(vi, smul(a.dt, acc_g));
     v3 x = vadd(xi, muls(v, a.dt));
-    const float sx = a.bound[0], sy = a.bound[1], sz = a.bound[2], d = a.damping;
-    if (x.x < -sx) { x.x = -sx; if (v.x < 0.0f) v.x = v.x * d; }
-    if (x.x > sx) { x.x = sx; if (v.x > 0.0f) v.x = v.x * d; }
-    if (x.y < -sy) { x.y = -sy; if (v.y < 0.0f) v.y = v.y * d; }
-    if (x.y > sy) { x.y = sy; if (v.y > 0.0f) v.y = v.y * d; }
-    if (x.z < 0.0f) { x.z = 0.0f; if (v.z < 0.0f) v.z = v.z * d; }
-    if (x.z > sz * 2.0f) { x.z = sz * 2.0f; if (v.z > 0.0f) v.z = v.z * d; }
+    const float sx = a.bound[0], sy = a.bound[1], sz = a.bound[2], dmp = a.damping;
+    if (x.x < -sx) { x.x = -sx; if (v.x < 0.0f) v.x = v.x * dmp; }
+    if (x.x > sx) { x.x = sx; if (v.x > 0.0f) v.x = v.x * dmp; }
+    if (x.y < -sy) { x.y = -sy; if (v.y < 0.0f) v.y = v.y * dmp; }
+    if (x.y > sy) { x.y = sy; if (v.y > 0.0f) v.y = v.y * dmp; }
+    if (x.z < 0.0f) { x.z = 0.0f; if (v.z < 0.0f) v.z = v.z * dmp; }
+    if (x.z > sz * 2.0f) { x.z = sz * 2.0f; if (v.z > 0.0f) v.z = v.z * dmp; }
     st3(a.v_out, i, v);
     st3(a.x_out, i, x);
 }
@@ -242,15 +272,18 @@ struct DfDensityArgs {
     float* factor;
     int* neighbor_count;
     float4* sca;   // rho, factor, kappa, kappa/max(rho,1e-6)
+    float4* vel;   // .w <- m/max(rho,1e-6)
 };
 
 // compute_density + compute_factor_dfsph  solver_dfsph.py:106-174 (one pass)
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_dfsph_density_factor(DfDensityArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
     const int s = pass_slot();
     if (s >= a.c.n) return;
     const int i = a.c.perm[s];
+    const float4 xi4 = a.c.pos[s];
     if (!(a.c.sflags[s] & kActive)) {
         a.rho[i] = 0.0f;
         a.factor[i] = 0.0f;
@@ -258,9 +291,10 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_density_factor(DfDensityAr
         float kw = a.kappa_warm ? a.kappa_warm[i] : 0.0f;
         if (a.kappa_warm) a.kappa_commit[i] = kw;
         a.sca[s] = make_float4(0.0f, 0.0f, kw, kw / fmax_(0.0f, 1.0e-6f));
+        a.vel[s].w = xi4.w / fmax_(0.0f, 1.0e-6f);
+        if (a.c.L.mode == kListBuild) a.c.L.cnt[s] = 0;
         return;
     }
-    const float4 xi4 = a.c.pos[s];
     const v3 xi = xyz(xi4);
     const float h = a.c.h;
     float rho_temp = 0.0f;
@@ -268,16 +302,16 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_density_factor(DfDensityAr
     float sum_grad_sq = 0.0f;
     int count = 0;
     auto counter = [&](int k) { count += (k != s && (a.c.sflags[k] & kActive)) ? 1 : 0; };
-    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, counter,
-                                            [&](int k) {
-                                                if (!(a.c.sflags[k] & kActive)) return;
-                                                const float4 xj = a.c.pos[k];
-                                                v3 r = vsub(xi, xyz(xj));
-                                                rho_temp += a.wvol[k] * cubic_w(vlen(r), h);
-                                                v3 grad = smul(xj.w, cubic_grad(r, h));
-                                                sum_grad = vadd(sum_grad, grad);
-                                                sum_grad_sq += vdot(grad, grad);
-                                            });
+    NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, counter, [&](int k) {
+        if (!(a.c.sflags[k] & kActive)) return;
+        const float4 xj = a.c.pos[k];
+        v3 r = vsub(xi, xyz(xj));
+        const float rn = vlen(r);
+        rho_temp += a.wvol[k] * cubic_w(rn, h);
+        v3 grad = smul(xj.w, cubic_grad_r(r, rn, h));
+        sum_grad = vadd(sum_grad, grad);
+        sum_grad_sq += vdot(grad, grad);
+    });
     const float denom = vdot(sum_grad, sum_grad) + sum_grad_sq;
     const float f = (count < a.min_neighbors || denom < 1.0e-6f) ? 0.0f : 1.0f / denom;
     a.rho[i] = rho_temp;
@@ -289,12 +323,12 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_density_factor(DfDensityAr
         a.kappa_commit[i] = kw;
     }
     a.sca[s] = make_float4(rho_temp, f, kw, kw / fmax_(rho_temp, 1.0e-6f));
+    a.vel[s].w = xi4.w / fmax_(rho_temp, 1.0e-6f);
 }
 
 struct DfNonPressureArgs {
     PassCommon c;
     const float4* __restrict__ vel;
-    const float4* __restrict__ sca;
     const float* __restrict__ surface_tension;
     const float* __restrict__ viscosity;
     const float* __restrict__ gravity;
@@ -307,9 +341,10 @@ struct DfNonPressureArgs {
 };
 
 // acceleration_non_pressure + predict_velocity  solver_dfsph.py:177-242
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
     const int s = pass_slot();
     if (s >= a.c.n) return;
     const int i = a.c.perm[s];
@@ -324,16 +359,25 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureA
     const v3 vi = xyz(a.vel[s]);
     const float m_i = xi4.w, mu = a.viscosity[i], gamma = a.surface_tension[i];
     const float h = a.c.h, pr = a.particle_radius;
+    const float visc_den0 = 0.01f * (h * h);
+    const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
-    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
-                                            [&](int k) {
-                                                if (k == s || !(a.c.sflags[k] & kActive)) return;
-                                                const float4 xj = a.c.pos[k];
-                                                v3 d = vsub(xi, xyz(xj));
-                                                float dist = vlen(d);
-                                                acc = acc_non_pressure(acc, a.sca[k].x, vi, xyz(a.vel[k]), m_i, xj.w,
-                                                                       mu, gamma, d, fmax_(dist, pr), dist, h);
-                                            });
+    if (gamma != 0.0f || mu != 0.0f) {
+        NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+            if (k == s || !(a.c.sflags[k] & kActive)) return;
+            const float4 xj = a.c.pos[k];
+            const v3 d = vsub(xi, xyz(xj));
+            const float dist = vlen(d);
+            if (gamma != 0.0f)
+                acc = vsub(acc, smul(gamma / m_i * xj.w, muls(d, cubic_w(fmax_(dist, pr), h))));
+            if (mu != 0.0f) {
+                const float4 vj = a.vel[k];
+                float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
+                float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
+                acc = vadd(acc, smul(sv, cubic_grad_r(d, dist, h)));
+            }
+        });
+    }
     int w = a.world[i];
     w = w > 0 ? w : 0;
     const v3 acc_g = vadd(acc, ld3(a.gravity, w));
@@ -350,8 +394,6 @@ struct DfRateArgs {
     float dt;
     float max_kappa;
     int density_loop;   // 1: density loop (rho_star, kappa), 0: divergence loop (kappa_v)
-    int iter;
-    double eta;
     float* d_rho_dt;
     float* rho_star;
     float4* sca;
@@ -362,9 +404,10 @@ struct DfRateArgs {
 // compute_density_change_rate (+ predict_density_star, compute_kappa_density
 // or compute_kappa_divergence, and the host residual)  solver_dfsph.py:245-306,
 // 356-372, 579-596
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
     const int stop_idx = a.density_loop ? kStopRho : kStopVel;
     if (a.control[stop_idx]) return;
     const int s = pass_slot();
@@ -384,13 +427,12 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
             const v3 vi = xyz(a.vstar_s[s]);
             const float h = a.c.h;
             float div_val = 0.0f;
-            for_each_neighbor<kPassBlock, kListCap>(
-                a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{}, [&](int k) {
-                    if (!(a.c.sflags[k] & kActive)) return;
-                    const float4 xj = a.c.pos[k];
-                    v3 grad_w = cubic_grad(vsub(xi, xyz(xj)), h);
-                    div_val += xj.w * vdot(vsub(vi, xyz(a.vstar_s[k])), grad_w);
-                });
+            NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+                if (!(a.c.sflags[k] & kActive)) return;
+                const float4 xj = a.c.pos[k];
+                v3 grad_w = cubic_grad(vsub(xi, xyz(xj)), h);
+                div_val += xj.w * vdot(vsub(vi, xyz(a.vstar_s[k])), grad_w);
+            });
             a.d_rho_dt[i] = div_val;
             const bool dyn = (fl & kDynamic) != 0;
             float kap = 0.0f;
@@ -416,7 +458,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
             a.sca[s] = si;
         }
     }
-    residual_reduce(res_v, res_c, a.partials, a.control, stop_idx, a.density_loop ? kItRho : kItVel, a.iter, a.eta);
+    residual_partials(res_v, res_c, a.partials);
 }
 
 struct DfCorrArgs {
@@ -433,9 +475,10 @@ struct DfCorrArgs {
 
 // apply_constant_density_correction / apply_divergence_free_correction
 // solver_dfsph.py:309-337, 375-403 (and the kappa write of :290-306, 356-372)
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_dfsph_correction(DfCorrArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
     if (a.stop_idx >= 0 && a.control[a.stop_idx]) return;
     const int s = pass_slot();
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.corr_idx >= 0) a.control[a.corr_idx] += 1;
@@ -448,13 +491,12 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_correction(DfCorrArgs a)
     const float h = a.c.h;
     const float kri = si.w;
     v3 dv = V(0.0f, 0.0f, 0.0f);
-    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
-                                            [&](int k) {
-                                                if (!(a.c.sflags[k] & kActive)) return;
-                                                const float4 xj = a.c.pos[k];
-                                                float coeff = xj.w * (kri + a.sca[k].w);
-                                                dv = vadd(dv, smul(coeff, cubic_grad(vsub(xi, xyz(xj)), h)));
-                                            });
+    NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+        if (!(a.c.sflags[k] & kActive)) return;
+        const float4 xj = a.c.pos[k];
+        float coeff = xj.w * (kri + a.sca[k].w);
+        dv = vadd(dv, smul(coeff, cubic_grad(vsub(xi, xyz(xj)), h)));
+    });
     const float4 vs4 = a.vstar_s[s];
     const v3 vs = vsub(xyz(vs4), smul(a.dt, dv));
     st3(a.v_star, i, vs);
@@ -555,34 +597,42 @@ struct PbfDensityArgs {
 };
 
 // compute_density (poly6)  solver_pbf.py:114-141; in the final pass also
-// update_velocity_from_positions :402-416 and, without XSPH, :939-954
+// update_velocity_from_positions :402-416 and, without XSPH, :939-954.
+// vel[s].w <- m/max(rho,1e-6) for the non-pressure and XSPH passes.
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_pbf_density(PbfDensityArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
+    PassCommon c = a.c;
     const int s = pass_slot();
-    if (s >= a.c.n) return;
-    const int i = a.c.perm[s];
-    const int fl = a.c.sflags[s];
-    const float4 xi4 = a.c.pos[s];
+    if (s >= c.n) return;
+    const int i = c.perm[s];
+    const int fl = c.sflags[s];
+    const float4 xi4 = c.pos[s];
     const v3 xi = xyz(xi4);
     float rho_temp = 0.0f;
     if (fl & kActive) {
-        const float h = a.c.h;
-        for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
-                                                [&](int k) {
-                                                    if (!(a.c.sflags[k] & kActive)) return;
-                                                    v3 r = vsub(xi, xyz(a.c.pos[k]));
-                                                    rho_temp += a.wvol[k] * poly6_w(vlen(r), h);
-                                                });
+        const float h = c.h;
+        NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+            if (!(c.sflags[k] & kActive)) return;
+            v3 r = vsub(xi, xyz(c.pos[k]));
+            rho_temp += a.wvol[k] * poly6_w(vlen(r), h);
+        });
+    } else if (c.L.mode == kListBuild) {
+        c.L.cnt[s] = 0;
     }
     a.rho[i] = rho_temp;
     a.sca[s] = make_float4(rho_temp, 0.0f, 0.0f, 0.0f);
-    if (!a.final_pass) return;
+    const float mr = xi4.w / fmax_(rho_temp, 1.0e-6f);
+    if (!a.final_pass) {
+        a.vel[s].w = mr;
+        return;
+    }
     v3 v;
     const bool dyn = (fl & kDynamic) != 0;
     if (dyn) v = vdivs(vsub(xi, ld3(a.x_in, i)), fmax_(a.dt, 1.0e-8f));
     else v = ld3(a.v_in, i);
-    a.vel[s] = make_float4(v.x, v.y, v.z, 0.0f);
+    a.vel[s] = make_float4(v.x, v.y, v.z, mr);
     if (a.finalize) {
         v3 x = xi;
         if (dyn) a.bounds.clamp_damp(x, v, a.damping);
@@ -594,7 +644,6 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_density(PbfDensityArgs a)
 struct PbfNonPressureArgs {
     PassCommon c;
     const float4* __restrict__ vel;
-    const float4* __restrict__ sca;
     const float* __restrict__ surface_tension;
     const float* __restrict__ viscosity;
     float particle_radius;
@@ -608,43 +657,54 @@ struct PbfNonPressureArgs {
 
 // acceleration_non_pressure (PBF) + apply_non_pressure_prediction +
 // apply_position_bounds  solver_pbf.py:144-189, 227-242, 784-796
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_pbf_nonpressure(PbfNonPressureArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
     const int s = pass_slot();
-    if (s >= a.c.n) return;
-    const int i = a.c.perm[s];
-    const float4 xi4 = a.c.pos[s];
-    if (!(a.c.sflags[s] & kDynamic)) {
-        st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
-        a.pos_next[s] = xi4;
-        return;
+    if (s < a.c.n) {
+        const int i = a.c.perm[s];
+        const float4 xi4 = a.c.pos[s];
+        if (!(a.c.sflags[s] & kDynamic)) {
+            st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
+            a.pos_next[s] = xi4;
+        } else {
+            const v3 xi = xyz(xi4);
+            const float4 vi4 = a.vel[s];
+            const v3 vi = xyz(vi4);
+            const float m_i = xi4.w, mu = a.viscosity[i], gamma = a.surface_tension[i];
+            const float h = a.c.h, pr = a.particle_radius;
+            const float visc_den0 = 0.01f * (h * h);
+            const float mu10 = 10.0f * mu;
+            v3 acc = V(0.0f, 0.0f, 0.0f);
+            if (gamma != 0.0f || mu != 0.0f) {
+                NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+                    if (k == s || !(a.c.sflags[k] & kActive)) return;
+                    const float4 xj = a.c.pos[k];
+                    const v3 d = vsub(xi, xyz(xj));
+                    const float dist = vlen(d);
+                    if (gamma != 0.0f)
+                        acc = vsub(acc, smul(gamma / m_i * xj.w, muls(d, poly6_w(fmax_(dist, pr), h))));
+                    if (mu != 0.0f) {
+                        const float4 vj = a.vel[k];
+                        float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
+                        if (v_current_nei != 0.0f) {
+                            float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
+                            acc = vadd(acc, smul(sv, spiky_grad_r(d, dist, h)));
+                        }
+                    }
+                });
+            }
+            st3(a.accel, i, acc);
+            const v3 delta_v = smul(a.dt, acc);
+            const v3 v = vadd(vi, delta_v);
+            v3 x = vadd(xi, smul(a.dt, delta_v));
+            x = a.bounds.clamp(x);
+            st3(a.v_pred, i, v);
+            st3(a.x_pred, i, x);
+            a.pos_next[s] = make_float4(x.x, x.y, x.z, xi4.w);
+        }
     }
-    const v3 xi = xyz(xi4);
-    const v3 vi = xyz(a.vel[s]);
-    const float m_i = xi4.w, mu = a.viscosity[i], gamma = a.surface_tension[i];
-    const float h = a.c.h, pr = a.particle_radius;
-    v3 acc = V(0.0f, 0.0f, 0.0f);
-    if (gamma != 0.0f || mu != 0.0f) {
-        for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
-                                                [&](int k) {
-                                                    if (k == s || !(a.c.sflags[k] & kActive)) return;
-                                                    const float4 xj = a.c.pos[k];
-                                                    v3 d = vsub(xi, xyz(xj));
-                                                    float dist = vlen(d);
-                                                    acc = pbf_acc_non_pressure(acc, a.sca[k].x, vi, xyz(a.vel[k]),
-                                                                               m_i, xj.w, mu, gamma, d,
-                                                                               fmax_(dist, pr), dist, h);
-                                                });
-    }
-    st3(a.accel, i, acc);
-    const v3 delta_v = smul(a.dt, acc);
-    const v3 v = vadd(vi, delta_v);
-    v3 x = vadd(xi, smul(a.dt, delta_v));
-    x = a.bounds.clamp(x);
-    st3(a.v_pred, i, v);
-    st3(a.x_pred, i, x);
-    a.pos_next[s] = make_float4(x.x, x.y, x.z, xi4.w);
 }
 
 struct PbfLambdaArgs {
@@ -659,13 +719,15 @@ struct PbfLambdaArgs {
 };
 
 // compute_pbf_lambda  solver_pbf.py:245-302
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_pbf_lambda(PbfLambdaArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
+    PassCommon c = a.c;
     const int s = pass_slot();
-    if (s >= a.c.n) return;
-    const int i = a.c.perm[s];
-    if (!(a.c.sflags[s] & kDynamic)) {
+    if (s >= c.n) return;
+    const int i = c.perm[s];
+    if (!(c.sflags[s] & kDynamic)) {
         a.pbf_lambda[i] = 0.0f;
         a.rho[i] = 0.0f;
         a.neighbor_count[i] = 0;
@@ -673,25 +735,24 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_lambda(PbfLambdaArgs a)
         return;
     }
     const float rho0_i = fmax_(a.rest_density[i], 1.0e-6f);
-    const v3 xi = xyz(a.c.pos[s]);
-    const float h = a.c.h;
+    const v3 xi = xyz(c.pos[s]);
+    const float h = c.h;
     float rho_i = 0.0f;
     v3 grad_i = V(0.0f, 0.0f, 0.0f);
     float sum_grad_sq = 0.0f;
     int count = 0;
-    for_each_neighbor<kPassBlock, kListCap>(a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{},
-                                            [&](int k) {
-                                                if (!(a.c.sflags[k] & kActive)) return;
-                                                const float4 xj = a.c.pos[k];
-                                                v3 xij = vsub(xi, xyz(xj));
-                                                float r = vlen(xij);
-                                                if (k != s && r < h) count += 1;
-                                                float mass_j = xj.w;
-                                                rho_i += mass_j * poly6_w(r, h);
-                                                v3 grad_j = smul(-(mass_j / rho0_i), spiky_grad(xij, h));
-                                                sum_grad_sq += vdot(grad_j, grad_j);
-                                                grad_i = vsub(grad_i, grad_j);
-                                            });
+    NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+        if (!(c.sflags[k] & kActive)) return;
+        const float4 xj = c.pos[k];
+        v3 xij = vsub(xi, xyz(xj));
+        float r = vlen(xij);
+        if (k != s && r < h) count += 1;
+        float mass_j = xj.w;
+        rho_i += mass_j * poly6_w(r, h);
+        v3 grad_j = smul(-(mass_j / rho0_i), spiky_grad_r(xij, r, h));
+        sum_grad_sq += vdot(grad_j, grad_j);
+        grad_i = vsub(grad_i, grad_j);
+    });
     sum_grad_sq += vdot(grad_i, grad_i);
     const float c_i = fmax_(rho_i / rho0_i - 1.0f, 0.0f);
     a.rho[i] = rho_i;
@@ -717,50 +778,52 @@ struct PbfDeltaArgs {
 
 // compute_delta_position + apply_delta_position (+ apply_position_bounds after
 // the last iteration)  solver_pbf.py:305-363, 860-872
+template <int CAP, bool SCORR>
 __global__ __launch_bounds__(kPassBlock) void k_pbf_delta(PbfDeltaArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
+    PassCommon c = a.c;
     const int s = pass_slot();
-    if (s >= a.c.n) return;
-    const int i = a.c.perm[s];
-    const float4 xi4 = a.c.pos[s];
-    if (!(a.c.sflags[s] & kDynamic)) {
-        st3(a.delta_p, i, V(0.0f, 0.0f, 0.0f));
-        a.pos_next[s] = xi4;
-        st3(a.x_pred, i, xyz(xi4));
-        return;
+    if (s < c.n) {
+        const int i = c.perm[s];
+        const float4 xi4 = c.pos[s];
+        if (!(c.sflags[s] & kDynamic)) {
+            st3(a.delta_p, i, V(0.0f, 0.0f, 0.0f));
+            a.pos_next[s] = xi4;
+            st3(a.x_pred, i, xyz(xi4));
+        } else {
+            const v3 xi = xyz(xi4);
+            const float rho0_i = fmax_(a.rest_density[i], 1.0e-6f);
+            const float lam_i = a.sca[s].x;
+            const float h = c.h;
+            v3 dp = V(0.0f, 0.0f, 0.0f);
+            NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+                if (k == s || !(c.sflags[k] & kActive)) return;
+                const float4 xj = c.pos[k];
+                v3 xij = vsub(xi, xyz(xj));
+                const float rn = vlen(xij);
+                v3 grad = spiky_grad_r(xij, rn, h);
+                float scorr = 0.0f;
+                if (SCORR) scorr = compute_scorr(rn, h, a.scorr_k, a.scorr_n, a.scorr_w_delta_q);
+                dp = vadd(dp, smul((lam_i + a.sca[k].x + scorr) * xj.w, grad));
+            });
+            dp = vdivs(dp, rho0_i);
+            if (a.max_delta_position > 0.0f) {
+                float dp_len = vlen(dp);
+                if (dp_len > a.max_delta_position) dp = muls(dp, a.max_delta_position / fmax_(dp_len, 1.0e-12f));
+            }
+            st3(a.delta_p, i, dp);
+            v3 x = vadd(xi, dp);
+            if (a.apply_bounds) x = a.bounds.clamp(x);
+            st3(a.x_pred, i, x);
+            a.pos_next[s] = make_float4(x.x, x.y, x.z, xi4.w);
+        }
     }
-    const v3 xi = xyz(xi4);
-    const float rho0_i = fmax_(a.rest_density[i], 1.0e-6f);
-    const float lam_i = a.sca[s].x;
-    const float h = a.c.h;
-    v3 dp = V(0.0f, 0.0f, 0.0f);
-    for_each_neighbor<kPassBlock, kListCap>(
-        a.c.g, a.c.pos, xi.x, xi.y, xi.z, h, a.c.cut2, lds, NoCount{}, [&](int k) {
-            if (k == s || !(a.c.sflags[k] & kActive)) return;
-            const float4 xj = a.c.pos[k];
-            v3 xij = vsub(xi, xyz(xj));
-            v3 grad = spiky_grad(xij, h);
-            float scorr = 0.0f;
-            if (a.scorr_k != 0.0f) scorr = compute_scorr(vlen(xij), h, a.scorr_k, a.scorr_n, a.scorr_w_delta_q);
-            dp = vadd(dp, smul((lam_i + a.sca[k].x + scorr) * xj.w, grad));
-        });
-    dp = vdivs(dp, rho0_i);
-    if (a.max_delta_position > 0.0f) {
-        float dp_len = vlen(dp);
-        if (dp_len > a.max_delta_position) dp = muls(dp, a.max_delta_position / fmax_(dp_len, 1.0e-12f));
-    }
-    st3(a.delta_p, i, dp);
-    v3 x = vadd(xi, dp);
-    if (a.apply_bounds) x = a.bounds.clamp(x);
-    st3(a.x_pred, i, x);
-    a.pos_next[s] = make_float4(x.x, x.y, x.z, xi4.w);
 }
 
 struct PbfXsphArgs {
     PassCommon c;
     const float4* __restrict__ vel;
-    const float4* __restrict__ sca;
     float xsph_c;
     float damping;
     PbfBounds bounds;
@@ -769,28 +832,30 @@ struct PbfXsphArgs {
 };
 
 // apply_xsph_viscosity + copies + apply_bounds  solver_pbf.py:434-463, 919-954
+template <int CAP>
 __global__ __launch_bounds__(kPassBlock) void k_pbf_xsph(PbfXsphArgs a)
 {
-    __shared__ int lds[kPassBlock * kListCap];
+    __shared__ int lds[kPassBlock * CAP];
+    PassCommon c = a.c;
     const int s = pass_slot();
-    if (s >= a.c.n) return;
-    const int i = a.c.perm[s];
-    v3 x = xyz(a.c.pos[s]);
+    if (s >= c.n) return;
+    const int i = c.perm[s];
+    v3 x = xyz(c.pos[s]);
     const v3 vi = xyz(a.vel[s]);
-    if (!(a.c.sflags[s] & kDynamic)) {
+    if (!(c.sflags[s] & kDynamic)) {
         st3(a.x_out, i, x);
         st3(a.v_out, i, vi);
         return;
     }
-    const float h = a.c.h;
+    const float h = c.h;
     v3 corr = V(0.0f, 0.0f, 0.0f);
-    for_each_neighbor<kPassBlock, kListCap>(
-        a.c.g, a.c.pos, x.x, x.y, x.z, h, a.c.cut2, lds, NoCount{}, [&](int k) {
-            if (k == s || !(a.c.sflags[k] & kActive)) return;
-            const float4 xj = a.c.pos[k];
-            float w = poly6_w(vlen(vsub(x, xyz(xj))), h);
-            corr = vadd(corr, muls(smul(xj.w / fmax_(a.sca[k].x, 1.0e-6f), vsub(xyz(a.vel[k]), vi)), w));
-        });
+    NEIGHBORS(c, s, x.x, x.y, x.z, NoCount{}, [&](int k) {
+        if (k == s || !(c.sflags[k] & kActive)) return;
+        const float4 xj = c.pos[k];
+        const float4 vj = a.vel[k];
+        float w = poly6_w(vlen(vsub(x, xyz(xj))), h);
+        corr = vadd(corr, muls(smul(vj.w, vsub(xyz(vj), vi)), w));
+    });
     v3 v = vadd(vi, smul(a.xsph_c, corr));
     a.bounds.clamp_damp(x, v, a.damping);
     st3(a.x_out, i, x);
@@ -801,7 +866,19 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_xsph(PbfXsphArgs a)
 // host orchestration
 // ---------------------------------------------------------------------------
 
-static PassCommon make_common(const fs_grid* g, const fs_work* w, const float4* pos, float h)
+static NbrList list_of(const fs_work* w, int n, int mode, float cut2)
+{
+    NbrList L;
+    L.idx = w->nbr;
+    L.cnt = w->nbr_count;
+    L.cap = w->list_capacity;
+    L.n = n;
+    L.mode = (w->list_capacity > 0 && w->nbr && w->nbr_count) ? mode : kListNone;
+    L.cut2 = cut2;
+    return L;
+}
+
+static PassCommon make_common(const fs_grid* g, const fs_work* w, const float4* pos, float h, int list_mode)
 {
     PassCommon c;
     c.g = grid_view(g);
@@ -811,6 +888,7 @@ static PassCommon make_common(const fs_grid* g, const fs_work* w, const float4* 
     c.sflags = w->flags;
     c.h = h;
     c.cut2 = h * h * kCutSlack;
+    c.L = list_of(w, g->num_points, list_mode, c.cut2);
     return c;
 }
 
@@ -833,7 +911,7 @@ static int32_t check_common(const fs_grid* g, const fs_work* w, const fs_model* 
     return FS_OK;
 }
 
-#define LAUNCH_PASS(kernel, nb, s, args) FS_LAUNCH(kernel, dim3(nb), dim3(kPassBlock), 0, s, args)
+#define LAUNCH_PASS(kernel, nb, s, args) FS_LAUNCH((kernel), dim3(nb), dim3(kPassBlock), 0, s, args)
 
 static GatherOp gather_for(const fs_model* m, fs_work* w, const float* x, const float* v, float4* vel_dst)
 {
@@ -857,22 +935,24 @@ using namespace fs;
 
 extern "C" {
 
-size_t fs_work_bytes(int32_t capacity)
+size_t fs_work_bytes(int32_t capacity, int32_t list_capacity)
 {
-    if (capacity < 0) return 0;
+    if (capacity < 0 || list_capacity < 0) return 0;
     size_t cap = (size_t)(capacity > 0 ? capacity : 1);
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t nblocks = (cap + kPassBlock - 1) / kPassBlock;
-    return al(cap * 16) * 4 + al(cap * 4) * 2 + al(cap * 12) * 2 + al(nblocks * 16 + 16) + al(16 * 4) + al(8 * 4);
+    return al(cap * 16) * 4 + al(cap * 4) * 2 + al(cap * 12) * 2 + al(cap * 4 * (size_t)list_capacity) +
+           al(cap * 4) + al(nblocks * 16 + 16) + al(kControlWords * 4) + al(8 * 4);
 }
 
-int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity, fs_stream_t stream)
+int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity, int32_t list_capacity,
+                     fs_stream_t stream)
 {
-    if (!w || !workspace || capacity < 0) {
+    if (!w || !workspace || capacity < 0 || list_capacity < 0) {
         set_error("fs_work_init: invalid arguments");
         return FS_ERR_ARG;
     }
-    size_t need = fs_work_bytes(capacity);
+    size_t need = fs_work_bytes(capacity, list_capacity);
     if (bytes < need) {
         set_error("fs_work_init: workspace %zu bytes < required %zu", bytes, need);
         return FS_ERR_WORKSPACE;
@@ -887,6 +967,7 @@ int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity
         return r;
     };
     w->capacity = capacity;
+    w->list_capacity = list_capacity;
     w->pos = (float*)take(cap * 16);
     w->pos_alt = (float*)take(cap * 16);
     w->vel = (float*)take(cap * 16);
@@ -895,11 +976,13 @@ int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity
     w->flags = (int32_t*)take(cap * 4);
     w->x_pred = (float*)take(cap * 12);
     w->v_pred = (float*)take(cap * 12);
+    w->nbr = list_capacity > 0 ? (int32_t*)take(cap * 4 * (size_t)list_capacity) : nullptr;
+    w->nbr_count = (int32_t*)take(cap * 4);
     w->partials = (double*)take(nblocks * 16 + 16);
-    w->control = (int32_t*)take(16 * 4);
+    w->control = (int32_t*)take(kControlWords * 4);
     w->sync = (uint32_t*)take(8 * 4);
     hipStream_t s = (hipStream_t)stream;
-    int32_t rc = hip_check(hipMemsetAsync(w->control, 0, 16 * 4, s), "fs_work_init memset");
+    int32_t rc = hip_check(hipMemsetAsync(w->control, 0, kControlWords * 4, s), "fs_work_init memset");
     if (rc) return rc;
     return hip_check(hipMemsetAsync(w->sync, 0, 8 * 4, s), "fs_work_init memset");
 }
@@ -928,10 +1011,9 @@ int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_
     if (rc) return rc;
     const int nb = blocks_for(n, kPassBlock);
     const float4* pos = reinterpret_cast<const float4*>(w->pos);
-    PassCommon c = make_common(g, w, pos, h);
 
     WcsphDensityArgs da;
-    da.c = c;
+    da.c = make_common(g, w, pos, h, kListBuild);
     da.wvol = w->wvol;
     da.rest_density = m->rest_density;
     da.stiffness = m->stiffness;
@@ -940,10 +1022,11 @@ int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_
     da.rho = io->rho;
     da.pressure = io->pressure;
     da.sca = reinterpret_cast<float4*>(w->sca);
-    LAUNCH_PASS(k_wcsph_density, nb, s, da);
+    da.vel = reinterpret_cast<float4*>(w->vel);
+    LAUNCH_PASS(k_wcsph_density<kScanCap>, nb, s, da);
 
     WcsphForceArgs fa;
-    fa.c = c;
+    fa.c = make_common(g, w, pos, h, kListUse);
     fa.vel = reinterpret_cast<const float4*>(w->vel);
     fa.sca = reinterpret_cast<const float4*>(w->sca);
     fa.surface_tension = m->surface_tension;
@@ -959,7 +1042,7 @@ int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_
     fa.accel = io->accel;
     fa.x_out = io->x_out;
     fa.v_out = io->v_out;
-    LAUNCH_PASS(k_wcsph_force, nb, s, fa);
+    LAUNCH_PASS(k_wcsph_force<kReplayCap>, nb, s, fa);
     return FS_OK;
 }
 
@@ -986,17 +1069,18 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     const int nb = blocks_for(n, kPassBlock);
     float4* vstar_s = reinterpret_cast<float4*>(w->pos_alt);
     float4* sca = reinterpret_cast<float4*>(w->sca);
+    float4* vel = reinterpret_cast<float4*>(w->vel);
 
     // build on x_in (solver_dfsph.py:624)
     PreOp pre;
     pre.x_in = io->x_in;
     pre.control_reset = w->control;
-    rc = build_grid(g, n, h, pre, gather_for(m, w, io->x_in, io->v_in, reinterpret_cast<float4*>(w->vel)), s);
+    rc = build_grid(g, n, h, pre, gather_for(m, w, io->x_in, io->v_in, vel), s);
     if (rc) return rc;
-    PassCommon c = make_common(g, w, reinterpret_cast<const float4*>(w->pos), h);
+    const float4* pos = reinterpret_cast<const float4*>(w->pos);
 
     DfDensityArgs dfa;
-    dfa.c = c;
+    dfa.c = make_common(g, w, pos, h, kListBuild);
     dfa.wvol = w->wvol;
     dfa.min_neighbors = prm->min_neighbors_for_factor;
     dfa.kappa_warm = prm->warm_start ? io->kappa_in : nullptr;
@@ -1005,12 +1089,13 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     dfa.factor = io->factor;
     dfa.neighbor_count = io->neighbor_count;
     dfa.sca = sca;
-    LAUNCH_PASS(k_dfsph_density_factor, nb, s, dfa);
+    dfa.vel = vel;
+    LAUNCH_PASS(k_dfsph_density_factor<kScanCap>, nb, s, dfa);
 
+    const PassCommon use = make_common(g, w, pos, h, kListUse);
     DfNonPressureArgs npa;
-    npa.c = c;
-    npa.vel = reinterpret_cast<const float4*>(w->vel);
-    npa.sca = sca;
+    npa.c = use;
+    npa.vel = vel;
     npa.surface_tension = m->surface_tension;
     npa.viscosity = m->viscosity;
     npa.gravity = m->gravity;
@@ -1020,10 +1105,10 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     npa.accel = io->accel;
     npa.v_star = io->v_star;
     npa.vstar_s = vstar_s;
-    LAUNCH_PASS(k_dfsph_nonpressure, nb, s, npa);
+    LAUNCH_PASS(k_dfsph_nonpressure<kReplayCap>, nb, s, npa);
 
     DfCorrArgs ca;
-    ca.c = c;
+    ca.c = use;
     ca.sca = sca;
     ca.dt = dt;
     ca.v_star = io->v_star;
@@ -1033,11 +1118,11 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
         ca.stop_idx = -1;
         ca.corr_idx = -1;
         ca.kappa_commit = nullptr;
-        LAUNCH_PASS(k_dfsph_correction, nb, s, ca);
+        LAUNCH_PASS(k_dfsph_correction<kReplayCap>, nb, s, ca);
     }
 
     DfRateArgs ra;
-    ra.c = c;
+    ra.c = use;
     ra.vstar_s = vstar_s;
     ra.rest_density = m->rest_density;
     ra.dt = dt;
@@ -1048,14 +1133,14 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     ra.control = w->control;
     ra.density_loop = 1;
     ra.max_kappa = prm->max_kappa;
-    ra.eta = (double)prm->eta_density;
     ca.stop_idx = kStopRho;
     ca.corr_idx = kCorrRho;
     ca.kappa_commit = io->kappa;
     for (int it = 0; it < prm->max_rho_iterations; ++it) {   // solver_dfsph.py:711-767
-        ra.iter = it;
-        LAUNCH_PASS(k_dfsph_rate, nb, s, ra);
-        LAUNCH_PASS(k_dfsph_correction, nb, s, ca);
+        LAUNCH_PASS(k_dfsph_rate<kReplayCap>, nb, s, ra);
+        FS_LAUNCH(k_dfsph_residual, dim3(1), dim3(kPassBlock), 0, s, w->partials, nb, w->control, kStopRho, kItRho,
+                  it, prm->eta_density);
+        LAUNCH_PASS(k_dfsph_correction<kReplayCap>, nb, s, ca);
     }
 
     // integrate_positions fused into the second build's hash (solver_dfsph.py:769-783)
@@ -1069,30 +1154,30 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     pre2.dt = dt;
     rc = build_grid(g, n, h, pre2, gather_for(m, w, io->x_out, io->v_star, vstar_s), s);
     if (rc) return rc;
-    c = make_common(g, w, reinterpret_cast<const float4*>(w->pos), h);
-    dfa.c = c;
+    dfa.c = make_common(g, w, pos, h, kListBuild);
     dfa.kappa_warm = prm->warm_start ? io->kappa_v_in : nullptr;
     dfa.kappa_commit = io->kappa_v;
-    LAUNCH_PASS(k_dfsph_density_factor, nb, s, dfa);   // solver_dfsph.py:785-812
+    LAUNCH_PASS(k_dfsph_density_factor<kScanCap>, nb, s, dfa);   // solver_dfsph.py:785-812
 
-    ca.c = c;
+    const PassCommon use2 = make_common(g, w, pos, h, kListUse);
+    ca.c = use2;
     if (prm->warm_start) {   // solver_dfsph.py:814-832
         ca.stop_idx = -1;
         ca.corr_idx = -1;
         ca.kappa_commit = nullptr;
-        LAUNCH_PASS(k_dfsph_correction, nb, s, ca);
+        LAUNCH_PASS(k_dfsph_correction<kReplayCap>, nb, s, ca);
     }
-    ra.c = c;
+    ra.c = use2;
     ra.density_loop = 0;
     ra.max_kappa = prm->max_kappa_v;
-    ra.eta = (double)prm->eta_divergence;
     ca.stop_idx = kStopVel;
     ca.corr_idx = kCorrVel;
     ca.kappa_commit = io->kappa_v;
     for (int it = 0; it < prm->max_vel_iterations; ++it) {   // solver_dfsph.py:834-883
-        ra.iter = it;
-        LAUNCH_PASS(k_dfsph_rate, nb, s, ra);
-        LAUNCH_PASS(k_dfsph_correction, nb, s, ca);
+        LAUNCH_PASS(k_dfsph_rate<kReplayCap>, nb, s, ra);
+        FS_LAUNCH(k_dfsph_residual, dim3(1), dim3(kPassBlock), 0, s, w->partials, nb, w->control, kStopVel, kItVel,
+                  it, prm->eta_divergence);
+        LAUNCH_PASS(k_dfsph_correction<kReplayCap>, nb, s, ca);
     }
 
     DfFinalArgs fa;
@@ -1131,6 +1216,7 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
     const float dt = prm->dt;
     const int nb = blocks_for(n, kPassBlock);
     float4* sca = reinterpret_cast<float4*>(w->sca);
+    float4* vel = reinterpret_cast<float4*>(w->vel);
     float4* bufs[2] = {reinterpret_cast<float4*>(w->pos), reinterpret_cast<float4*>(w->pos_alt)};
     int cur = 0;
     PbfBounds bounds;
@@ -1153,11 +1239,14 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
     for (int d = 0; d < 3; ++d) pre.bound[d] = prm->bound[d];
     pre.particle_radius = prm->particle_radius;
     pre.boundary_epsilon = prm->boundary_epsilon;
-    rc = build_grid(g, n, h, pre, gather_for(m, w, w->x_pred, w->v_pred, reinterpret_cast<float4*>(w->vel)), s);
+    rc = build_grid(g, n, h, pre, gather_for(m, w, w->x_pred, w->v_pred, vel), s);
     if (rc) return rc;
 
+    // Positions change after the non-pressure pass and after every
+    // delta-position pass: the pass that follows a change scans (and
+    // records), the one after it, at the same positions, replays.
     PbfDensityArgs da;
-    da.c = make_common(g, w, bufs[cur], h);
+    da.c = make_common(g, w, bufs[cur], h, kListBuild);
     da.wvol = w->wvol;
     da.rho = io->rho;
     da.sca = sca;
@@ -1168,15 +1257,14 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
     da.dt = dt;
     da.damping = prm->boundary_damping;
     da.bounds = bounds;
-    da.vel = reinterpret_cast<float4*>(w->vel);
+    da.vel = vel;
     da.x_out = io->x_out;
     da.v_out = io->v_out;
-    LAUNCH_PASS(k_pbf_density, nb, s, da);   // :738-751
+    LAUNCH_PASS(k_pbf_density<kScanCap>, nb, s, da);   // :738-751
 
     PbfNonPressureArgs na;
-    na.c = make_common(g, w, bufs[cur], h);
-    na.vel = reinterpret_cast<const float4*>(w->vel);
-    na.sca = sca;
+    na.c = make_common(g, w, bufs[cur], h, kListUse);
+    na.vel = vel;
     na.surface_tension = m->surface_tension;
     na.viscosity = m->viscosity;
     na.particle_radius = prm->particle_radius;
@@ -1186,12 +1274,12 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
     na.x_pred = w->x_pred;
     na.v_pred = w->v_pred;
     na.pos_next = bufs[cur ^ 1];
-    LAUNCH_PASS(k_pbf_nonpressure, nb, s, na);   // :752-796
+    LAUNCH_PASS(k_pbf_nonpressure<kReplayCap>, nb, s, na);   // :752-796
     cur ^= 1;
 
     for (int it = 0; it < prm->max_iterations; ++it) {   // :800-858
         PbfLambdaArgs la;
-        la.c = make_common(g, w, bufs[cur], h);
+        la.c = make_common(g, w, bufs[cur], h, kListBuild);
         la.rest_density = m->rest_density;
         la.lambda_epsilon = prm->lambda_epsilon;
         la.min_neighbors = prm->min_neighbors_for_lambda;
@@ -1199,10 +1287,10 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
         la.rho = io->rho;
         la.neighbor_count = io->neighbor_count;
         la.sca = sca;
-        LAUNCH_PASS(k_pbf_lambda, nb, s, la);
+        LAUNCH_PASS(k_pbf_lambda<kScanCap>, nb, s, la);
 
         PbfDeltaArgs pa;
-        pa.c = make_common(g, w, bufs[cur], h);
+        pa.c = make_common(g, w, bufs[cur], h, kListUse);
         pa.sca = sca;
         pa.rest_density = m->rest_density;
         pa.scorr_k = prm->scorr_k;
@@ -1214,7 +1302,8 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
         pa.delta_p = io->delta_p;
         pa.x_pred = w->x_pred;
         pa.pos_next = bufs[cur ^ 1];
-        LAUNCH_PASS(k_pbf_delta, nb, s, pa);
+        if (prm->scorr_k != 0.0f) LAUNCH_PASS((k_pbf_delta<kReplayCap, true>), nb, s, pa);
+        else LAUNCH_PASS((k_pbf_delta<kReplayCap, false>), nb, s, pa);
         cur ^= 1;
         if (prm->rebuild_grid_each_iteration) {
             PreOp rp;
@@ -1226,21 +1315,20 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
     }
 
     const bool xsph = prm->xsph_c > 0.0f;
-    da.c = make_common(g, w, bufs[cur], h);
+    da.c = make_common(g, w, bufs[cur], h, kListBuild);
     da.final_pass = 1;
     da.finalize = xsph ? 0 : 1;
-    LAUNCH_PASS(k_pbf_density, nb, s, da);   // :878-917 (+ :939-954 without XSPH)
+    LAUNCH_PASS(k_pbf_density<kScanCap>, nb, s, da);   // :878-917 (+ :939-954 without XSPH)
     if (xsph) {
         PbfXsphArgs xa;
-        xa.c = make_common(g, w, bufs[cur], h);
-        xa.vel = reinterpret_cast<const float4*>(w->vel);
-        xa.sca = sca;
+        xa.c = make_common(g, w, bufs[cur], h, kListUse);
+        xa.vel = vel;
         xa.xsph_c = prm->xsph_c;
         xa.damping = prm->boundary_damping;
         xa.bounds = bounds;
         xa.x_out = io->x_out;
         xa.v_out = io->v_out;
-        LAUNCH_PASS(k_pbf_xsph, nb, s, xa);   // :919-954
+        LAUNCH_PASS(k_pbf_xsph<kReplayCap>, nb, s, xa);   // :919-954
     }
     return FS_OK;
 }

@@ -37,6 +37,10 @@ class ParticleSolver(SolverBase):
 
     _name = "Solver"
 
+    #: neighbour-list entries reserved per particle (lists replayed by the
+    #: passes that follow a grid build; overflowing particles scan the grid)
+    neighbor_list_capacity = 96
+
     def __init__(self, model, config):
         super().__init__(model=model)
         self.config = config
@@ -50,11 +54,12 @@ class ParticleSolver(SolverBase):
         self._particle_volume = Array.full(n, 0.0, float32, model.device)
         self._bounds = (float(config.bound_width), float(config.bound_height), float(config.bound_length))
         lib = _lib.load()
-        nbytes = lib.fs_work_bytes(max(n, 1))
+        cap_list = int(self.neighbor_list_capacity)
+        nbytes = lib.fs_work_bytes(max(n, 1), cap_list)
         self._work_mem = torch.empty(int(nbytes), dtype=torch.uint8, device=model.device)
         self._work = _lib.FsWork()
         _lib.check(lib.fs_work_init(ctypes.byref(self._work), ctypes.c_void_p(self._work_mem.data_ptr()), nbytes,
-                                    max(n, 1), ctypes.c_void_p(self._stream())), "fs_work_init")
+                                    max(n, 1), cap_list, ctypes.c_void_p(self._stream())), "fs_work_init")
 
     @property
     def smoothing_length(self) -> float:

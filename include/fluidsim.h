@@ -146,6 +146,7 @@ typedef struct fs_model {
  * gathered into the grid's sorted order, plus reduction scratch. */
 typedef struct fs_work {
     int32_t capacity;
+    int32_t list_capacity; /* neighbour-list entries per particle (0: no lists) */
     float* pos;         /* [cap*4] x,y,z,mass  sorted                          */
     float* pos_alt;     /* [cap*4] ping-pong positions (PBF)                    */
     float* vel;         /* [cap*4] vx,vy,vz,-  sorted                          */
@@ -154,13 +155,19 @@ typedef struct fs_work {
     int32_t* flags;     /* [cap]   flags (bit0 active, bit1 dynamic), sorted   */
     float* x_pred;      /* [cap*3] original order (PBF x_pred / DFSPH scratch)  */
     float* v_pred;      /* [cap*3] original order (PBF v_pred)                 */
+    int32_t* nbr;       /* [list_capacity*cap] neighbour lists (entry-major)    */
+    int32_t* nbr_count; /* [cap] list lengths (list_capacity+1 = overflowed)    */
     double* partials;   /* residual partial sums                               */
-    int32_t* control;   /* [16] loop control: stop flags, iteration counters   */
+    int32_t* control;   /* [64] loop control: stop flags, iteration counters   */
     uint32_t* sync;     /* [8]  self-resetting counters                        */
 } fs_work;
 
-size_t fs_work_bytes(int32_t capacity);
-int32_t fs_work_init(fs_work* work, void* workspace, size_t workspace_bytes, int32_t capacity, fs_stream_t stream);
+/* list_capacity: entries per particle of the neighbour lists recorded by the
+ * first pass after each grid build and replayed by later passes over the
+ * same candidates (0 disables them; every pass then scans the grid). */
+size_t fs_work_bytes(int32_t capacity, int32_t list_capacity);
+int32_t fs_work_init(fs_work* work, void* workspace, size_t workspace_bytes, int32_t capacity,
+                     int32_t list_capacity, fs_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* WCSPH                                                                    */

@@ -167,7 +167,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert set(declared) == set(_lib.SIGNATURES)
     assert lib.fs_version() >= 100
     assert lib.fs_grid_workspace_bytes(1000, 128, 128, 128) > 8 * 128 ** 3    # start + count per bucket
-    assert lib.fs_work_bytes(1000) > 0
+    assert lib.fs_work_bytes(1000, 96) > lib.fs_work_bytes(1000, 0) > 0
 
 
 def test_struct_layouts_agree_with_header(tmp_path):
