@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true", help="run only the PMC copy calibration kernel")
     ap.add_argument("--micro", default="", help="comma-separated particle counts for the grid/density/force "
                                                 "microbenchmark (e.g. 4000000,16000000)")
+    ap.add_argument("--batched", type=int, default=0, help="run only the batched form over this many scenes "
+                                                           "(64 for the batched workload)")
     return ap.parse_args()
 
 
@@ -198,6 +200,99 @@ def micro(sizes, reps):
             torch.cuda.empty_cache()
 
 
+def batched(num_scenes, steps, warmup, chunk=10):
+    """configs: 64 dam-break scenes of log-uniform size in one batch.  Timed:
+    (a) batched substeps with the per-scene CFL dt on the device, replayed from
+    a HIP graph of ``chunk`` substeps; (b) the same eagerly; (c) 64 sequential
+    plain ``SolverWCSPH.step`` calls per substep, one per scene, with each
+    scene's dt taken from the batch; (d) those 64 plain steps captured in one
+    graph.  Every variant advances every scene by one substep per step."""
+    import torch
+
+    from fluid_simulation_amd.batched import BatchedWCSPH, make_batch_scenes
+    from fluid_simulation_amd.examples.dam_break.simulation import DamBreakSimulation
+
+    steps = max(chunk, (steps + chunk - 1) // chunk * chunk)
+    scenes = make_batch_scenes(num_scenes, seed=3)
+    total = sum(s.count for s in scenes)
+    batch = BatchedWCSPH(scenes, device="cuda:0")
+    batch.begin_frame()
+    for _ in range(warmup):
+        batch.substep()
+    batch.check_status()
+    batch.remaining.fill_(1.0e9)          # no scene reaches the end of its time inside the timed region
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.substep()
+    torch.cuda.synchronize()
+    eager = (time.perf_counter() - t0) / steps
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            for _ in range(chunk):
+                batch.substep()
+    torch.cuda.current_stream().wait_stream(side)
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // chunk):
+        graph.replay()
+    torch.cuda.synchronize()
+    graphed = (time.perf_counter() - t0) / steps
+    batch.check_status()
+    dts = batch.dts()
+    del graph
+
+    sims = [DamBreakSimulation(s, "wcsph", device="cuda:0") for s in scenes]
+    for _ in range(2):
+        for sim, dt in zip(sims, dts):
+            sim._substep(float(dt))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sim, dt in zip(sims, dts):
+            sim._substep(float(dt))
+    torch.cuda.synchronize()
+    seq_eager = (time.perf_counter() - t0) / steps
+
+    # the 64 plain solves of one substep, captured once; buffers swap inside
+    # the capture, so capture two substeps to return to the starting buffers
+    graph = torch.cuda.CUDAGraph()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            for _ in range(2):
+                for sim, dt in zip(sims, dts):
+                    sim._substep(float(dt))
+    torch.cuda.current_stream().wait_stream(side)
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // 2):
+        graph.replay()
+    torch.cuda.synchronize()
+    seq_graph = (time.perf_counter() - t0) / (2 * (steps // 2))
+    for sim in sims:
+        sim.check_finite()
+    out = {"batched": {"scenes": num_scenes, "particles": total, "min_scene": min(s.count for s in scenes),
+                       "max_scene": max(s.count for s in scenes), "buckets": batch.num_buckets, "steps": steps,
+                       "ms_per_substep_graph": graphed * 1e3, "ms_per_substep_eager": eager * 1e3,
+                       "ms_per_substep_sequential_eager": seq_eager * 1e3,
+                       "ms_per_substep_sequential_graph": seq_graph * 1e3,
+                       "particle_substeps_per_s_graph": total / graphed,
+                       "particle_substeps_per_s_sequential_eager": total / seq_eager,
+                       "speedup_vs_sequential_eager": seq_eager / graphed,
+                       "speedup_vs_sequential_graph": seq_graph / graphed}}
+    del graph, sims, batch
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -205,6 +300,9 @@ def main():
         return
     if args.micro:
         micro([int(x) for x in args.micro.split(",")], args.steps)
+        return
+    if args.batched:
+        print(json.dumps(batched(args.batched, args.steps, args.warmup)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path does not shard "

@@ -96,6 +96,23 @@ class FsPbfIO(ctypes.Structure):
     ]
 
 
+class FsScene(ctypes.Structure):
+    _fields_ = [
+        ("particle_offset", c_int32), ("particle_count", c_int32), ("bucket_base", c_int32), ("dims", c_int32 * 3),
+        ("smoothing_length", c_float), ("cell_width_inv", c_float), ("particle_radius", c_float),
+        ("bound", c_float * 3), ("boundary_damping", c_float), ("dt", c_float),
+        ("clamp_negative_pressure", c_int32), ("pad_", c_int32),
+    ]
+
+
+class FsBatchIO(ctypes.Structure):
+    _fields_ = [
+        ("x_in", c_void_p), ("v_in", c_void_p), ("x_out", c_void_p), ("v_out", c_void_p),
+        ("rho", c_void_p), ("pressure", c_void_p), ("accel", c_void_p), ("scene_of", c_void_p),
+        ("scene_sorted", c_void_p),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/fluidsim.h
 SIGNATURES = {
     "fs_last_error": (ctypes.c_char_p, []),
@@ -119,6 +136,10 @@ SIGNATURES = {
                                 POINTER(FsDfsphIO), c_void_p]),
     "fs_pbf_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
                               POINTER(FsPbfIO), c_void_p]),
+    "fs_batch_wcsph_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), c_void_p, c_int32,
+                                      POINTER(FsBatchIO), c_void_p]),
+    "fs_batch_cfl_dt": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_double, c_double, c_double,
+                                  c_double, c_double, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fs_norm_maxima": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),
     "fs_check_finite": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),
 }

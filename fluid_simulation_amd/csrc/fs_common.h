@@ -201,8 +201,9 @@ struct GridView {
     int dx, dy, dz, dxy;
     int round_mode;
     int npts;                       // points in the build (slot ranges are clamped to it)
+    int base;                       // first bucket of this (scene's) table
     float inv;
-    const int* __restrict__ start;  // [dx*dy*dz + 1]
+    const int* __restrict__ start;  // [buckets + 1]
 };
 
 __device__ __forceinline__ int bucket_of(const GridView& g, float x, float y, float z)
@@ -210,7 +211,7 @@ __device__ __forceinline__ int bucket_of(const GridView& g, float x, float y, fl
     int cx = cell_coord(x * g.inv, g.round_mode);
     int cy = cell_coord(y * g.inv, g.round_mode);
     int cz = cell_coord(z * g.inv, g.round_mode);
-    return hash_axis(cz, g.dz) * g.dxy + hash_axis(cy, g.dy) * g.dx + hash_axis(cx, g.dx);
+    return g.base + hash_axis(cz, g.dz) * g.dxy + hash_axis(cy, g.dy) * g.dx + hash_axis(cx, g.dx);
 }
 
 // Iterator over the sorted-slot ranges visited by hash_grid_query(p, r):
@@ -257,7 +258,7 @@ struct QueryRows {
     }
     __device__ __forceinline__ void load_row(const GridView& g)
     {
-        row_base = hash_axis(zs + zi, g.dz) * g.dxy + hash_axis(ys + yi, g.dy) * g.dx;
+        row_base = g.base + hash_axis(zs + zi, g.dz) * g.dxy + hash_axis(ys + yi, g.dy) * g.dx;
         int a = row_base + (part == 0 ? hx0 : 0);
         int b = a + (part == 0 ? seg1 : seg2);
         k = min(max(g.start[a], 0), g.npts);

@@ -69,6 +69,71 @@ __global__ __launch_bounds__(kRedBlock) void k_check_finite(const float* __restr
 }
 
 // few blocks: each ends with one same-address atomic, which serialise
+// Per-scene maxima of |v|^2 and |a|^2: blockIdx.y picks the scene, blocks along
+// x stride over its particles; one atomicMax per block and quantity.
+__global__ __launch_bounds__(kRedBlock) void k_scene_maxima(const fs_scene* __restrict__ scenes,
+                                                            const float* __restrict__ v, const float* __restrict__ a,
+                                                            int use_accel, uint32_t* out)
+{
+    __shared__ float s_m[2][kRedBlock / 64];
+    const fs_scene sc = scenes[blockIdx.y];
+    float m0 = 0.0f, m1 = 0.0f;
+    for (int k = blockIdx.x * kRedBlock + threadIdx.x; k < sc.particle_count; k += gridDim.x * kRedBlock) {
+        const int i = sc.particle_offset + k;
+        m0 = fmax_(m0, norm2_or_inf(ld3(v, i)));
+        if (use_accel) m1 = fmax_(m1, norm2_or_inf(ld3(a, i)));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        m0 = fmax_(m0, __shfl_xor(m0, off, 64));
+        m1 = fmax_(m1, __shfl_xor(m1, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        s_m[0][wid] = m0;
+        s_m[1][wid] = m1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kRedBlock / 64; ++w) {
+            m0 = fmax_(m0, s_m[0][w]);
+            m1 = fmax_(m1, s_m[1][w]);
+        }
+        atomicMax(&out[2 * blockIdx.y], __float_as_uint(m0));
+        atomicMax(&out[2 * blockIdx.y + 1], __float_as_uint(m1));
+    }
+}
+
+// DamBreakSimulation.step's dt policy per scene  (simulation.py:289-310)
+__global__ void k_scene_dt(fs_scene* scenes, int num_scenes, uint32_t* maxima, double stiffness, double exponent,
+                           double rest_density, double viscosity, double frame_dt, int substeps, double* remaining,
+                           int* status)
+{
+    const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sidx >= num_scenes) return;
+    const float v2 = __uint_as_float(maxima[2 * sidx]);
+    const float a2 = __uint_as_float(maxima[2 * sidx + 1]);
+    maxima[2 * sidx] = 0u;
+    maxima[2 * sidx + 1] = 0u;
+    double rem = remaining[sidx];
+    double dt = 0.0;
+    if (rem > 1.0e-12) {
+        if (!isfinite(v2) || !isfinite(a2)) atomicMax(status, 1);
+        const double speed = sqrt((double)v2);
+        const double accel_max = sqrt((double)a2);
+        const double h = (double)scenes[sidx].smoothing_length;
+        double limit = frame_dt / substeps;
+        const double sound_speed = sqrt(stiffness * exponent / rest_density);
+        limit = fmin(limit, 0.25 * h / (sound_speed + speed));
+        if (accel_max > 0.0) limit = fmin(limit, 0.25 * sqrt(h / accel_max));
+        if (viscosity > 0.0) limit = fmin(limit, 0.025 * h * h / viscosity);
+        dt = fmin(rem, limit);
+        if (dt < 1.0e-10) atomicMax(status, 2);
+        remaining[sidx] = rem - dt;
+    }
+    scenes[sidx].dt = (float)dt;
+}
+
 static int red_blocks(int n)
 {
     int b = blocks_for(n, kRedBlock * 8);
@@ -90,12 +155,26 @@ int32_t fs_norm_maxima(const float* v, int32_t nv, const float* a, int32_t na, f
     }
     hipStream_t s = (hipStream_t)stream;
     uint32_t* out = reinterpret_cast<uint32_t*>(maxima);
-    hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, s, out, 2);
-    FS_LAUNCH_CHECK("k_fill_zero");
+    FS_LAUNCH(k_fill_zero, dim3(1), dim3(64), 0, s, out, 2);
     int n = nv > na ? nv : na;
     if (n == 0) return FS_OK;
-    hipLaunchKernelGGL(k_norm_maxima, dim3(red_blocks(n)), dim3(kRedBlock), 0, s, v, nv, a, na, out);
-    FS_LAUNCH_CHECK("k_norm_maxima");
+    FS_LAUNCH(k_norm_maxima, dim3(red_blocks(n)), dim3(kRedBlock), 0, s, v, nv, a, na, out);
+    return FS_OK;
+}
+
+int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, const float* v, const float* a, int32_t use_accel,
+                        double stiffness, double exponent, double rest_density, double viscosity, double frame_dt,
+                        int32_t substeps, double* remaining, uint32_t* scratch, int32_t* status, fs_stream_t stream)
+{
+    if (!scenes || num_scenes <= 0 || num_scenes > 65535 || !v || (use_accel && !a) || !remaining || !scratch ||
+        !status || substeps <= 0) {
+        set_error("fs_batch_cfl_dt: invalid arguments");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    FS_LAUNCH(k_scene_maxima, dim3(8, num_scenes), dim3(kRedBlock), 0, s, scenes, v, a, use_accel, scratch);
+    FS_LAUNCH(k_scene_dt, dim3(blocks_for(num_scenes, 64)), dim3(64), 0, s, scenes, num_scenes, scratch, stiffness,
+              exponent, rest_density, viscosity, frame_dt, substeps, remaining, status);
     return FS_OK;
 }
 
@@ -108,11 +187,9 @@ int32_t fs_check_finite(const float* x, const float* v, int32_t n, const float* 
     }
     hipStream_t s = (hipStream_t)stream;
     uint32_t* out = reinterpret_cast<uint32_t*>(invalid);
-    hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, s, out, 1);
-    FS_LAUNCH_CHECK("k_fill_zero");
+    FS_LAUNCH(k_fill_zero, dim3(1), dim3(64), 0, s, out, 1);
     if (n == 0) return FS_OK;
-    hipLaunchKernelGGL(k_check_finite, dim3(red_blocks(n)), dim3(kRedBlock), 0, s, x, v, n, density, nd, out);
-    FS_LAUNCH_CHECK("k_check_finite");
+    FS_LAUNCH(k_check_finite, dim3(red_blocks(n)), dim3(kRedBlock), 0, s, x, v, n, density, nd, out);
     return FS_OK;
 }
 

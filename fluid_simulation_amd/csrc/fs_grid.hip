@@ -102,6 +102,7 @@ GridView grid_view(const fs_grid* g)
     v.dxy = g->dim_x * g->dim_y;
     v.round_mode = g->round_mode;
     v.npts = g->num_points;
+    v.base = 0;
     v.inv = g->cell_width_inv;
     v.start = g->cell_start;
     return v;
@@ -158,7 +159,21 @@ __device__ __forceinline__ int hash_point(const HashArgs& a, int i)
         st3(p.v_out, i, v);
         st3(p.x_out, i, x);
     }
-    const int c = bucket_of(a.g, x.x, x.y, x.z);
+    int c;
+    if (p.scenes) {
+        // segmented grid: the point's scene selects cell width, dims and bucket base
+        const fs_scene& sc = p.scenes[p.scene_of[i]];
+        GridView g = a.g;
+        g.dx = sc.dims[0];
+        g.dy = sc.dims[1];
+        g.dz = sc.dims[2];
+        g.dxy = sc.dims[0] * sc.dims[1];
+        g.base = sc.bucket_base;
+        g.inv = sc.cell_width_inv;
+        c = bucket_of(g, x.x, x.y, x.z);
+    } else {
+        c = bucket_of(a.g, x.x, x.y, x.z);
+    }
     a.cell_of[i] = c;
     return c;
 }
@@ -343,6 +358,7 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int* __r
         int fl = ga.flags[i];
         int act = fl & 1;
         ga.sflags[f] = act | ((act && ga.mask[i] == 0) ? kDynamic : 0);
+        if (ga.scene_of) ga.sscene[f] = ga.scene_of[i];
         if (ga.v) {
             v3 u = ld3(ga.v, i);
             ga.vel[f] = make_float4(u.x, u.y, u.z, 0.0f);

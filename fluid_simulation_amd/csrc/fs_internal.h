@@ -30,6 +30,8 @@ struct PreOp {
     float particle_radius = 0.0f;
     float boundary_epsilon = 0.0f;
     int* control_reset = nullptr;  // zeroed by the hash kernel when non-null
+    const fs_scene* scenes = nullptr;   // segmented grid: per-scene cell width, dims, bucket base
+    const int* scene_of = nullptr;      // [n] scene of each point
 };
 
 // Particle data copied into sorted order by the final build stage.
@@ -44,6 +46,8 @@ struct GatherOp {
     float4* vel = nullptr;         // vx,vy,vz,0   (when v != nullptr)
     float* wvol = nullptr;         // rest_density * volume
     int* sflags = nullptr;         // bit0 active, bit1 dynamic
+    const int* scene_of = nullptr; // optional: scene ids, gathered to sscene
+    int* sscene = nullptr;
 };
 
 // Full build: hash (+pre-op), bucket scan, scatter, stable fix-up (+gather).

@@ -48,6 +48,20 @@ struct PassCommon {
 
 __device__ __forceinline__ int pass_slot() { return xcd_tile() * kPassBlock + threadIdx.x; }
 
+// Per-lane view of a batch scene: its bucket table, cell width and radius.
+__device__ __forceinline__ void scene_view(PassCommon& c, const fs_scene& sc)
+{
+    c.g.dx = sc.dims[0];
+    c.g.dy = sc.dims[1];
+    c.g.dz = sc.dims[2];
+    c.g.dxy = sc.dims[0] * sc.dims[1];
+    c.g.base = sc.bucket_base;
+    c.g.inv = sc.cell_width_inv;
+    c.h = sc.smoothing_length;
+    c.cut2 = c.h * c.h * kCutSlack;
+    c.L.cut2 = c.cut2;
+}
+
 struct NoCount {
     __device__ __forceinline__ void operator()(int) const {}
 };
@@ -129,6 +143,8 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_residual(const double* __r
 
 struct WcsphDensityArgs {
     PassCommon c;
+    const fs_scene* __restrict__ scenes;   // batch: scene table and sorted scene ids
+    const int* __restrict__ sscene;
     const float* __restrict__ wvol;
     const float* __restrict__ rest_density;
     const float* __restrict__ stiffness;
@@ -141,33 +157,40 @@ struct WcsphDensityArgs {
 };
 
 // compute_density_pressure  solver_wcsph.py:119-158
-template <int CAP>
+template <int CAP, bool BATCH>
 __global__ __launch_bounds__(kPassBlock) void k_wcsph_density(WcsphDensityArgs a)
 {
     __shared__ int lds[kPassBlock * CAP];
     const int s = pass_slot();
     if (s >= a.c.n) return;
-    const int i = a.c.perm[s];
-    const float4 xi = a.c.pos[s];
-    if (!(a.c.sflags[s] & kActive)) {
+    PassCommon c = a.c;
+    int clamp = a.clamp;
+    if (BATCH) {
+        const fs_scene& sc = a.scenes[a.sscene[s]];
+        scene_view(c, sc);
+        clamp = sc.clamp_negative_pressure;
+    }
+    const int i = c.perm[s];
+    const float4 xi = c.pos[s];
+    if (!(c.sflags[s] & kActive)) {
         a.rho[i] = 0.0f;
         a.pressure[i] = 0.0f;
         a.sca[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         a.vel[s].w = xi.w / fmax_(0.0f, 1.0e-6f);
-        if (a.c.L.mode == kListBuild) a.c.L.cnt[s] = 0;
+        if (c.L.mode == kListBuild) c.L.cnt[s] = 0;
         return;
     }
-    const float h = a.c.h;
+    const float h = c.h;
     float rho_temp = 0.0f;
-    NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-        if (!(a.c.sflags[k] & kActive)) return;
-        v3 dist = vsub(xyz(xi), xyz(a.c.pos[k]));
+    NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+        if (!(c.sflags[k] & kActive)) return;
+        v3 dist = vsub(xyz(xi), xyz(c.pos[k]));
         float mass_nei = a.wvol[k];
         rho_temp += mass_nei * cubic_w(vlen(dist), h);
     });
     const float rho_i = rho_temp;
     float p = a.stiffness[i] * (fpow_(rho_i / fmax_(a.rest_density[i], 1.0e-6f), a.exponent[i]) - 1.0f);
-    if (a.clamp) p = fmax_(p, 0.0f);
+    if (clamp) p = fmax_(p, 0.0f);
     a.rho[i] = rho_i;
     a.pressure[i] = p;
     a.sca[s] = make_float4(rho_i, p, p / fmax_(rho_i * rho_i, 1.0e-6f), 0.0f);
@@ -176,6 +199,8 @@ __global__ __launch_bounds__(kPassBlock) void k_wcsph_density(WcsphDensityArgs a
 
 struct WcsphForceArgs {
     PassCommon c;
+    const fs_scene* __restrict__ scenes;
+    const int* __restrict__ sscene;
     const float4* __restrict__ vel;
     const float4* __restrict__ sca;
     const float* __restrict__ surface_tension;
@@ -200,32 +225,45 @@ struct WcsphForceArgs {
 // p/max(rho^2,1e-6), m/max(rho,1e-6) precomputed -- the same float operations
 // on the same operands.  A zero cohesion (gamma) or viscosity (mu) coefficient
 // contributes an exact zero, so its term is skipped.
-template <int CAP>
+template <int CAP, bool BATCH>
 __global__ __launch_bounds__(kPassBlock) void k_wcsph_force(WcsphForceArgs a)
 {
     __shared__ int lds[kPassBlock * CAP];
     const int s = pass_slot();
     if (s >= a.c.n) return;
-    const int i = a.c.perm[s];
-    if (!(a.c.sflags[s] & kDynamic)) {
+    PassCommon c = a.c;
+    float pr = a.particle_radius, dt = a.dt, dmp = a.damping;
+    float sx = a.bound[0], sy = a.bound[1], sz = a.bound[2];
+    if (BATCH) {
+        const fs_scene& sc = a.scenes[a.sscene[s]];
+        scene_view(c, sc);
+        pr = sc.particle_radius;
+        dt = sc.dt;
+        dmp = sc.boundary_damping;
+        sx = sc.bound[0];
+        sy = sc.bound[1];
+        sz = sc.bound[2];
+    }
+    const int i = c.perm[s];
+    if (!(c.sflags[s] & kDynamic)) {
         st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
         st3(a.v_out, i, ld3(a.v_in, i));
         st3(a.x_out, i, ld3(a.x_in, i));
         return;
     }
-    const float4 xi4 = a.c.pos[s];
+    const float4 xi4 = c.pos[s];
     const v3 xi = xyz(xi4);
     const v3 vi = xyz(a.vel[s]);
     const float4 si = a.sca[s];
     const float dp_i = si.z, m_i = xi4.w;
     const float mu = a.viscosity[i], gamma = a.surface_tension[i];
-    const float h = a.c.h, pr = a.particle_radius;
+    const float h = c.h;
     const float visc_den0 = 0.01f * (h * h);
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
-    NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-        if (k == s || !(a.c.sflags[k] & kActive)) return;
-        const float4 xj = a.c.pos[k];
+    NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+        if (k == s || !(c.sflags[k] & kActive)) return;
+        const float4 xj = c.pos[k];
         const float4 sj = a.sca[k];
         const v3 d = vsub(xi, xyz(xj));
         const float distance = vlen(d);
@@ -245,9 +283,8 @@ __global__ __launch_bounds__(kPassBlock) void k_wcsph_force(WcsphForceArgs a)
     w = w > 0 ? w : 0;
     const v3 acc_g = vadd(acc, ld3(a.gravity, w));
     st3(a.accel, i, acc_g);
-    v3 v = vadd(vi, smul(a.dt, acc_g));
-    v3 x = vadd(xi, muls(v, a.dt));
-    const float sx = a.bound[0], sy = a.bound[1], sz = a.bound[2], dmp = a.damping;
+    v3 v = vadd(vi, smul(dt, acc_g));
+    v3 x = vadd(xi, muls(v, dt));
     if (x.x < -sx) { x.x = -sx; if (v.x < 0.0f) v.x = v.x * dmp; }
     if (x.x > sx) { x.x = sx; if (v.x > 0.0f) v.x = v.x * dmp; }
     if (x.y < -sy) { x.y = -sy; if (v.y < 0.0f) v.y = v.y * dmp; }
@@ -1023,7 +1060,9 @@ int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_
     da.pressure = io->pressure;
     da.sca = reinterpret_cast<float4*>(w->sca);
     da.vel = reinterpret_cast<float4*>(w->vel);
-    LAUNCH_PASS(k_wcsph_density<kScanCap>, nb, s, da);
+    da.scenes = nullptr;
+    da.sscene = nullptr;
+    LAUNCH_PASS((k_wcsph_density<kScanCap, false>), nb, s, da);
 
     WcsphForceArgs fa;
     fa.c = make_common(g, w, pos, h, kListUse);
@@ -1042,7 +1081,77 @@ int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_
     fa.accel = io->accel;
     fa.x_out = io->x_out;
     fa.v_out = io->v_out;
-    LAUNCH_PASS(k_wcsph_force<kReplayCap>, nb, s, fa);
+    fa.scenes = nullptr;
+    fa.sscene = nullptr;
+    LAUNCH_PASS((k_wcsph_force<kReplayCap, false>), nb, s, fa);
+    return FS_OK;
+}
+
+int32_t fs_batch_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_scene* scenes, int32_t num_scenes,
+                            const fs_batch_io* io, fs_stream_t stream)
+{
+    int32_t rc = check_common(g, w, m);
+    if (rc) return rc;
+    if (!scenes || num_scenes <= 0 || !io) {
+        set_error("fs_batch_wcsph_step: null scene table or io");
+        return FS_ERR_ARG;
+    }
+    const int n = m->n;
+    if (n == 0) return FS_OK;
+    if (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->pressure || !io->accel ||
+        !io->scene_of || !io->scene_sorted || !m->stiffness || !m->exponent) {
+        set_error("fs_batch_wcsph_step: missing arrays");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    PreOp pre;
+    pre.x_in = io->x_in;
+    pre.scenes = scenes;
+    pre.scene_of = io->scene_of;
+    GatherOp ga = gather_for(m, w, io->x_in, io->v_in, reinterpret_cast<float4*>(w->vel));
+    ga.scene_of = io->scene_of;
+    ga.sscene = io->scene_sorted;
+    // the radius only sets the (unused) uniform cell width of the segmented table
+    rc = build_grid(g, n, 1.0f, pre, ga, s);
+    if (rc) return rc;
+    const int nb = blocks_for(n, kPassBlock);
+    const float4* pos = reinterpret_cast<const float4*>(w->pos);
+
+    WcsphDensityArgs da;
+    da.c = make_common(g, w, pos, 1.0f, kListBuild);
+    da.scenes = scenes;
+    da.sscene = io->scene_sorted;
+    da.wvol = w->wvol;
+    da.rest_density = m->rest_density;
+    da.stiffness = m->stiffness;
+    da.exponent = m->exponent;
+    da.clamp = 1;
+    da.rho = io->rho;
+    da.pressure = io->pressure;
+    da.sca = reinterpret_cast<float4*>(w->sca);
+    da.vel = reinterpret_cast<float4*>(w->vel);
+    LAUNCH_PASS((k_wcsph_density<kScanCap, true>), nb, s, da);
+
+    WcsphForceArgs fa;
+    fa.c = make_common(g, w, pos, 1.0f, kListUse);
+    fa.scenes = scenes;
+    fa.sscene = io->scene_sorted;
+    fa.vel = reinterpret_cast<const float4*>(w->vel);
+    fa.sca = reinterpret_cast<const float4*>(w->sca);
+    fa.surface_tension = m->surface_tension;
+    fa.viscosity = m->viscosity;
+    fa.gravity = m->gravity;
+    fa.world = m->world;
+    fa.x_in = io->x_in;
+    fa.v_in = io->v_in;
+    fa.particle_radius = 0.0f;
+    fa.dt = 0.0f;
+    for (int d = 0; d < 3; ++d) fa.bound[d] = 0.0f;
+    fa.damping = 0.0f;
+    fa.accel = io->accel;
+    fa.x_out = io->x_out;
+    fa.v_out = io->v_out;
+    LAUNCH_PASS((k_wcsph_force<kReplayCap, true>), nb, s, fa);
     return FS_OK;
 }
 

@@ -285,6 +285,67 @@ int32_t fs_pbf_step(fs_grid* grid, fs_work* work, const fs_model* model, const f
                     const fs_pbf_io* io, fs_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Batched WCSPH: many independent scenes in one launch per pass            */
+/* ------------------------------------------------------------------------ */
+
+/* One scene of a batch (device array of these; no counterpart in the
+ * reference, which solves one scene per solver).  Particles of scene s are the
+ * contiguous range [particle_offset, particle_offset + particle_count) of the
+ * batch arrays; its buckets are [bucket_base, bucket_base + dims[0]*dims[1]*
+ * dims[2]) of one segmented table.  With dims[a] = 128 or dims[a] >= the
+ * scene's cell span + 4, a scene's candidate sets and visit order equal those
+ * of a plain 128^3 grid, so each scene evolves exactly as a plain solve. */
+typedef struct fs_scene {
+    int32_t particle_offset;
+    int32_t particle_count;
+    int32_t bucket_base;
+    int32_t dims[3];
+    float smoothing_length;       /* h: query radius and cell width        */
+    float cell_width_inv;         /* float32 1/h, as the plain build forms */
+    float particle_radius;
+    float bound[3];
+    float boundary_damping;
+    float dt;                     /* this scene's substep; 0 freezes it     */
+    int32_t clamp_negative_pressure;
+    int32_t pad_;
+} fs_scene;
+
+typedef struct fs_batch_io {
+    const float* x_in;
+    const float* v_in;
+    float* x_out;
+    float* v_out;
+    float* rho;
+    float* pressure;
+    float* accel;
+    const int32_t* scene_of;      /* [n] scene id of each particle          */
+    int32_t* scene_sorted;        /* [n] scratch: scene ids in sorted order */
+} fs_batch_io;
+
+/* One WCSPH substep of every scene of the batch: one segmented grid build
+ * and the two neighbour passes of fs_wcsph_step, each a single launch over
+ * all particles of all scenes.  `grid` must be initialised with dim_x = total
+ * buckets, dim_y = dim_z = 1. */
+int32_t fs_batch_wcsph_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_scene* scenes,
+                            int32_t num_scenes, const fs_batch_io* io, fs_stream_t stream);
+
+/* Per-scene CFL substep of the dam-break driver (simulation.py:289-310), on
+ * the device, in float64 as the reference's host code evaluates it:
+ *   limit = min(frame_dt/substeps, 0.25h/(c0+vmax), 0.25 sqrt(h/amax) if
+ *           amax > 0, 0.025 h^2/viscosity if viscosity > 0),
+ *   c0 = sqrt(stiffness*exponent/rest_density),
+ *   scenes[s].dt = min(remaining[s], limit) while remaining[s] > 1e-12
+ *   (else 0), remaining[s] -= dt,
+ * with vmax/amax the scene's maxima of |v| / |a| (a ignored when use_accel is
+ * 0).  status[0] = 1 if a maximum is non-finite, 2 if a scene's dt falls
+ * below 1e-10 (the reference's RuntimeError cases).  scratch: device uint32
+ * [2*num_scenes], zero on first use and left zero. */
+int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, const float* v, const float* a, int32_t use_accel,
+                        double stiffness, double exponent, double rest_density, double viscosity, double frame_dt,
+                        int32_t substeps, double* remaining, uint32_t* scratch, int32_t* status,
+                        fs_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Driver diagnostics  (examples/dam_break/simulation.py:142-176)           */
 /* ------------------------------------------------------------------------ */
 
