@@ -59,6 +59,13 @@ class FsWcsphIO(ctypes.Structure):
     ]
 
 
+class FsWcsphAdjointIO(ctypes.Structure):
+    _fields_ = [
+        ("forward", FsWcsphIO), ("xbar_out", c_void_p), ("vbar_out", c_void_p), ("xbar_in", c_void_p),
+        ("vbar_in", c_void_p), ("grad_stiffness", c_void_p), ("grad_viscosity", c_void_p), ("scratch", c_void_p),
+    ]
+
+
 class FsDfsphParams(ctypes.Structure):
     _fields_ = [
         ("eta_density", c_double), ("eta_divergence", c_double),
@@ -132,6 +139,9 @@ SIGNATURES = {
     "fs_work_init": (c_int32, [POINTER(FsWork), c_void_p, c_size_t, c_int32, c_int32, c_void_p]),
     "fs_wcsph_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsWcsphParams),
                                 POINTER(FsWcsphIO), c_void_p]),
+    "fs_wcsph_adjoint_bytes": (c_size_t, [c_int32]),
+    "fs_wcsph_adjoint_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsWcsphParams),
+                                        POINTER(FsWcsphAdjointIO), c_size_t, c_void_p]),
     "fs_dfsph_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsDfsphParams),
                                 POINTER(FsDfsphIO), c_void_p]),
     "fs_pbf_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),

@@ -103,6 +103,27 @@ __device__ __forceinline__ v3 cubic_grad_r(v3 r, float r_norm, float h)
 }
 __device__ __forceinline__ v3 cubic_grad(v3 r, float h) { return cubic_grad_r(r, vlen(r), h); }
 
+// H(r) u with H = d(cubic_grad)/dr on cubic_grad's branches (zero where it is
+// zero): cubic_grad = g(d) r, H u = g u + (g'(d)/d)(r.u) r, with
+//   q <= 0.5: g = 6k q(3q-2)/(d h),  g' = 18k/h^3
+//   q >  0.5: g = -6k(1-q)^2/(d h),  g' = 6k/h (2(1-q)/(h d) + (1-q)^2/d^2)
+__device__ __forceinline__ v3 cubic_hess_mul(v3 r, float r_norm, float h, v3 u)
+{
+    float k = 8.0f / (kPi * (h * h * h));
+    float q = r_norm / h;
+    if (!(r_norm > 1.0e-5f && q <= 1.0f)) return V(0.0f, 0.0f, 0.0f);
+    float g, gpd;
+    if (q <= 0.5f) {
+        g = 6.0f * k * q * (3.0f * q - 2.0f) / (r_norm * h);
+        gpd = 18.0f * k / (h * h * h) / r_norm;
+    } else {
+        float t = 1.0f - q;
+        g = -6.0f * k * t * t / (r_norm * h);
+        gpd = 6.0f * k / h * (2.0f * t / (h * r_norm) + t * t / (r_norm * r_norm)) / r_norm;
+    }
+    return vadd(smul(g, u), smul(gpd * vdot(r, u), r));
+}
+
 // PBF get_cubic = poly6  solver_pbf.py:23-32
 __device__ __forceinline__ float poly6_w(float r_norm, float h)
 {

@@ -296,6 +296,207 @@ __global__ __launch_bounds__(kPassBlock) void k_wcsph_force(WcsphForceArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// WCSPH adjoint: reverse-mode derivative of one substep (surface tension 0),
+// in gather form so that every particle's adjoint is one lane's sum over its
+// recorded neighbour list, in visit order, with no atomics.
+// ---------------------------------------------------------------------------
+
+struct WadjIntegrateArgs {
+    int n;
+    const int* __restrict__ perm;
+    const int* __restrict__ sflags;
+    const float* __restrict__ x_in;
+    const float* __restrict__ v_in;
+    const float* __restrict__ accel;
+    const float* __restrict__ xbar_out;
+    const float* __restrict__ vbar_out;
+    const float* __restrict__ viscosity;
+    float dt;
+    float bound[3];
+    float damping;
+    float4* abar;   // sorted: dL/da, 10*viscosity
+    float4* xbar;   // sorted partial dL/dx_in
+    float4* vbar;   // sorted partial dL/dv_in
+};
+
+// reverse of integrate_velocity, integrate_positions, apply_bounds
+// (solver_wcsph.py:231-305)
+__global__ __launch_bounds__(kPassBlock) void k_wadj_integrate(WadjIntegrateArgs a)
+{
+    const int s = pass_slot();
+    if (s >= a.n) return;
+    const int i = a.perm[s];
+    const v3 xb = ld3(a.xbar_out, i), vb = ld3(a.vbar_out, i);
+    const float c = 10.0f * a.viscosity[i];
+    if (!(a.sflags[s] & kDynamic)) {
+        a.xbar[s] = make_float4(xb.x, xb.y, xb.z, 0.0f);
+        a.vbar[s] = make_float4(vb.x, vb.y, vb.z, 0.0f);
+        a.abar[s] = make_float4(0.0f, 0.0f, 0.0f, c);
+        return;
+    }
+    const v3 u = vadd(ld3(a.v_in, i), smul(a.dt, ld3(a.accel, i)));
+    const v3 p = vadd(ld3(a.x_in, i), muls(u, a.dt));
+    const float lo[3] = {-a.bound[0], -a.bound[1], 0.0f};
+    const float hi[3] = {a.bound[0], a.bound[1], a.bound[2] * 2.0f};
+    const float pc[3] = {p.x, p.y, p.z}, uc[3] = {u.x, u.y, u.z};
+    float xbc[3] = {xb.x, xb.y, xb.z}, vbc[3] = {vb.x, vb.y, vb.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (pc[d] < lo[d]) {
+            xbc[d] = 0.0f;
+            if (uc[d] < 0.0f) vbc[d] = vbc[d] * a.damping;
+        } else if (pc[d] > hi[d]) {
+            xbc[d] = 0.0f;
+            if (uc[d] > 0.0f) vbc[d] = vbc[d] * a.damping;
+        }
+    }
+    const v3 xbu = V(xbc[0], xbc[1], xbc[2]);
+    const v3 vbu = vadd(V(vbc[0], vbc[1], vbc[2]), smul(a.dt, xbu));
+    const v3 ab = smul(a.dt, vbu);
+    a.xbar[s] = make_float4(xbu.x, xbu.y, xbu.z, 0.0f);
+    a.vbar[s] = make_float4(vbu.x, vbu.y, vbu.z, 0.0f);
+    a.abar[s] = make_float4(ab.x, ab.y, ab.z, c);
+}
+
+struct WadjForceArgs {
+    PassCommon c;
+    const float4* __restrict__ vel;    // v, m/max(rho,1e-6)
+    const float4* __restrict__ sca;    // rho, p, p/max(rho^2,1e-6), -
+    const float4* __restrict__ abar;
+    const float* __restrict__ rest_density;
+    const float* __restrict__ stiffness;
+    const float* __restrict__ exponent;
+    int clamp;
+    float4* xbar;
+    float4* vbar;
+    float* rhobar;                     // sorted dL/drho
+    float* grad_stiffness;             // original order, +=
+    float* grad_viscosity;
+};
+
+// reverse of acceleration_wcsph (pressure + viscosity) and of the EOS
+// (solver_wcsph.py:92-106,119-228): per particle k, the pairs (k,j) of its
+// own acceleration and (j,k) where it is the neighbour.
+template <int CAP>
+__global__ __launch_bounds__(kPassBlock) void k_wadj_force(WadjForceArgs a)
+{
+    __shared__ int lds[kPassBlock * CAP];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const PassCommon& c = a.c;
+    if (!(c.sflags[s] & kActive)) {
+        a.rhobar[s] = 0.0f;
+        return;
+    }
+    const int i = c.perm[s];
+    const float4 xk4 = c.pos[s];
+    const float4 vk4 = a.vel[s];
+    const float4 sk = a.sca[s];
+    const float4 ak4 = a.abar[s];
+    const v3 xk = xyz(xk4), vk = xyz(vk4), ak = xyz(ak4);
+    const float rho_k = sk.x, p_k = sk.y, P_k = sk.z, m_k = xk4.w, w_k = vk4.w, c_k = ak4.w;
+    const float h = c.h;
+    const float eps2 = 0.01f * (h * h);
+    v3 xb = V(0.0f, 0.0f, 0.0f), vb = V(0.0f, 0.0f, 0.0f);
+    float Pb = 0.0f, wb = 0.0f, mub = 0.0f;
+    NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, [&](int j) {
+        if (j == s || !(c.sflags[j] & kActive)) return;
+        const float4 xj4 = c.pos[j];
+        const v3 r = vsub(xk, xyz(xj4));
+        const float d = vlen(r);
+        if (!(d / h <= 1.0f)) return;
+        const v3 G = cubic_grad_r(r, d, h);
+        const float4 aj4 = a.abar[j];
+        const float4 vj4 = a.vel[j];
+        const float4 sj = a.sca[j];
+        const v3 aj = xyz(aj4);
+        const float m_j = xj4.w;
+        const float P_j = sj.z;
+        const float w_j = vj4.w;
+        const float c_j = aj4.w;
+        const float den = d * d + eps2;
+        const v3 dv = vsub(vk, xyz(vj4));
+        const float sv = vdot(dv, r) / den;
+        const float Gak = vdot(G, ak);
+        const float Gaj = vdot(G, aj);
+        const v3 Hak = cubic_hess_mul(r, d, h, ak);
+        const v3 Haj = cubic_hess_mul(r, d, h, aj);
+        const v3 ds = vdivs(vsub(dv, smul(2.0f * sv, r)), den);
+        xb = vadd(xb, smul(-m_j * (P_k + P_j), Hak));
+        xb = vadd(xb, smul(m_k * (P_j + P_k), Haj));
+        Pb += -m_j * Gak + m_k * Gaj;
+        const float ck_wj = c_k * w_j;
+        const float cj_wk = c_j * w_k;
+        xb = vadd(xb, smul(ck_wj, vadd(smul(Gak, ds), smul(sv, Hak))));
+        xb = vsub(xb, smul(cj_wk, vadd(smul(Gaj, ds), smul(sv, Haj))));
+        vb = vadd(vb, smul((ck_wj * Gak - cj_wk * Gaj) / den, r));
+        mub += 10.0f * w_j * sv * Gak;
+        wb += -c_j * sv * Gaj;
+    });
+    float rb = 0.0f;
+    const float pb = Pb / fmax_(rho_k * rho_k, 1.0e-6f);
+    if (rho_k * rho_k > 1.0e-6f) rb += Pb * (-2.0f * p_k / (rho_k * rho_k * rho_k));
+    if (rho_k > 1.0e-6f) rb += wb * (-m_k / (rho_k * rho_k));
+    const float rd = fmax_(a.rest_density[i], 1.0e-6f);
+    const float ratio = rho_k / rd;
+    const float gam = a.exponent[i];
+    const float B = a.stiffness[i];
+    const float p_raw = B * (fpow_(ratio, gam) - 1.0f);
+    float sb = 0.0f;
+    if (!a.clamp || p_raw > 0.0f) {
+        rb += pb * B * gam * fpow_(ratio, gam - 1.0f) / rd;
+        sb = pb * (fpow_(ratio, gam) - 1.0f);
+    }
+    a.rhobar[s] = rb;
+    const float4 x0 = a.xbar[s], v0 = a.vbar[s];
+    a.xbar[s] = make_float4(x0.x + xb.x, x0.y + xb.y, x0.z + xb.z, 0.0f);
+    a.vbar[s] = make_float4(v0.x + vb.x, v0.y + vb.y, v0.z + vb.z, 0.0f);
+    a.grad_stiffness[i] += sb;
+    a.grad_viscosity[i] += mub;
+}
+
+struct WadjDensityArgs {
+    PassCommon c;
+    const float* __restrict__ wvol;
+    const float* __restrict__ rhobar;
+    const float4* __restrict__ xbar_s;
+    const float4* __restrict__ vbar_s;
+    float* xbar_in;
+    float* vbar_in;
+};
+
+// reverse of the density sum of compute_density_pressure (solver_wcsph.py:119-158)
+// and the final write of dL/dx_in, dL/dv_in
+template <int CAP>
+__global__ __launch_bounds__(kPassBlock) void k_wadj_density(WadjDensityArgs a)
+{
+    __shared__ int lds[kPassBlock * CAP];
+    const int s = pass_slot();
+    if (s >= a.c.n) return;
+    const PassCommon& c = a.c;
+    const int i = c.perm[s];
+    const float4 x0 = a.xbar_s[s];
+    v3 xb = V(0.0f, 0.0f, 0.0f);
+    if (c.sflags[s] & kActive) {
+        const float4 xk4 = c.pos[s];
+        const v3 xk = xyz(xk4);
+        const float wv_k = a.wvol[s];
+        const float rb_k = a.rhobar[s];
+        const float h = c.h;
+        NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, [&](int j) {
+            if (j == s || !(c.sflags[j] & kActive)) return;
+            const v3 r = vsub(xk, xyz(c.pos[j]));
+            const float d = vlen(r);
+            if (!(d / h <= 1.0f)) return;
+            xb = vadd(xb, smul(rb_k * a.wvol[j] + a.rhobar[j] * wv_k, cubic_grad_r(r, d, h)));
+        });
+    }
+    st3(a.xbar_in, i, V(x0.x + xb.x, x0.y + xb.y, x0.z + xb.z));
+    const float4 v0 = a.vbar_s[s];
+    st3(a.vbar_in, i, V(v0.x, v0.y, v0.z));
+}
+
+// ---------------------------------------------------------------------------
 // DFSPH
 // ---------------------------------------------------------------------------
 
@@ -1024,23 +1225,17 @@ int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity
     return hip_check(hipMemsetAsync(w->sync, 0, 8 * 4, s), "fs_work_init memset");
 }
 
-int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm, const fs_wcsph_io* io,
-                      fs_stream_t stream)
+}  // extern "C"
+
+namespace fs {
+
+// grid build + density/EOS pass (records neighbour lists) + force/integrate
+// pass (replays them): SolverWCSPH.step  solver_wcsph.py:431-535
+static int32_t wcsph_forward(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
+                             const fs_wcsph_io* io, hipStream_t s)
 {
-    int32_t rc = check_common(g, w, m);
-    if (rc) return rc;
-    if (!prm || !io) {
-        set_error("fs_wcsph_step: null params or io");
-        return FS_ERR_ARG;
-    }
+    int32_t rc;
     const int n = m->n;
-    if (n == 0) return FS_OK;
-    if (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->pressure || !io->accel ||
-        !m->stiffness || !m->exponent) {
-        set_error("fs_wcsph_step: missing arrays");
-        return FS_ERR_ARG;
-    }
-    hipStream_t s = (hipStream_t)stream;
     const float h = prm->smoothing_length;
     PreOp pre;
     pre.x_in = io->x_in;
@@ -1084,6 +1279,132 @@ int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_
     fa.scenes = nullptr;
     fa.sscene = nullptr;
     LAUNCH_PASS((k_wcsph_force<kReplayCap, false>), nb, s, fa);
+    return FS_OK;
+}
+
+static int32_t check_wcsph(const fs_grid* g, const fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
+                           const fs_wcsph_io* io, const char* who)
+{
+    int32_t rc = check_common(g, w, m);
+    if (rc) return rc;
+    if (!prm || !io) {
+        set_error("%s: null params or io", who);
+        return FS_ERR_ARG;
+    }
+    if (m->n > 0 && (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->pressure ||
+                     !io->accel || !m->stiffness || !m->exponent)) {
+        set_error("%s: missing arrays", who);
+        return FS_ERR_ARG;
+    }
+    return FS_OK;
+}
+
+static size_t adjoint_scratch_bytes(size_t cap)
+{
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    return al(cap * 16) * 3 + al(cap * 4);
+}
+
+}  // namespace fs
+
+extern "C" {
+
+int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm, const fs_wcsph_io* io,
+                      fs_stream_t stream)
+{
+    int32_t rc = check_wcsph(g, w, m, prm, io, "fs_wcsph_step");
+    if (rc) return rc;
+    if (m->n == 0) return FS_OK;
+    return wcsph_forward(g, w, m, prm, io, (hipStream_t)stream);
+}
+
+size_t fs_wcsph_adjoint_bytes(int32_t capacity)
+{
+    if (capacity < 0) return 0;
+    return adjoint_scratch_bytes((size_t)(capacity > 0 ? capacity : 1));
+}
+
+int32_t fs_wcsph_adjoint_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
+                              const fs_wcsph_adjoint_io* io, size_t scratch_bytes, fs_stream_t stream)
+{
+    if (!io) {
+        set_error("fs_wcsph_adjoint_step: null io");
+        return FS_ERR_ARG;
+    }
+    int32_t rc = check_wcsph(g, w, m, prm, &io->forward, "fs_wcsph_adjoint_step");
+    if (rc) return rc;
+    const int n = m->n;
+    if (n == 0) return FS_OK;
+    if (!io->xbar_out || !io->vbar_out || !io->xbar_in || !io->vbar_in || !io->grad_stiffness ||
+        !io->grad_viscosity || !io->scratch) {
+        set_error("fs_wcsph_adjoint_step: missing adjoint arrays");
+        return FS_ERR_ARG;
+    }
+    if (scratch_bytes < adjoint_scratch_bytes((size_t)n)) {
+        set_error("fs_wcsph_adjoint_step: scratch %zu bytes < required %zu", scratch_bytes,
+                  adjoint_scratch_bytes((size_t)n));
+        return FS_ERR_WORKSPACE;
+    }
+    if (w->list_capacity <= 0) {
+        set_error("fs_wcsph_adjoint_step: the workspace needs neighbour lists (list_capacity > 0)");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    rc = wcsph_forward(g, w, m, prm, &io->forward, s);
+    if (rc) return rc;
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    char* p = (char*)io->scratch;
+    float4* abar = (float4*)p;
+    float4* xbar = (float4*)(p + al((size_t)n * 16));
+    float4* vbar = (float4*)(p + 2 * al((size_t)n * 16));
+    float* rhobar = (float*)(p + 3 * al((size_t)n * 16));
+    const int nb = blocks_for(n, kPassBlock);
+    const float h = prm->smoothing_length;
+    const float4* pos = reinterpret_cast<const float4*>(w->pos);
+
+    WadjIntegrateArgs ia;
+    ia.n = n;
+    ia.perm = g->point_ids;
+    ia.sflags = w->flags;
+    ia.x_in = io->forward.x_in;
+    ia.v_in = io->forward.v_in;
+    ia.accel = io->forward.accel;
+    ia.xbar_out = io->xbar_out;
+    ia.vbar_out = io->vbar_out;
+    ia.viscosity = m->viscosity;
+    ia.dt = prm->dt;
+    for (int d = 0; d < 3; ++d) ia.bound[d] = prm->bound[d];
+    ia.damping = prm->boundary_damping;
+    ia.abar = abar;
+    ia.xbar = xbar;
+    ia.vbar = vbar;
+    LAUNCH_PASS(k_wadj_integrate, nb, s, ia);
+
+    WadjForceArgs fa;
+    fa.c = make_common(g, w, pos, h, kListUse);
+    fa.vel = reinterpret_cast<const float4*>(w->vel);
+    fa.sca = reinterpret_cast<const float4*>(w->sca);
+    fa.abar = abar;
+    fa.rest_density = m->rest_density;
+    fa.stiffness = m->stiffness;
+    fa.exponent = m->exponent;
+    fa.clamp = prm->clamp_negative_pressure;
+    fa.xbar = xbar;
+    fa.vbar = vbar;
+    fa.rhobar = rhobar;
+    fa.grad_stiffness = io->grad_stiffness;
+    fa.grad_viscosity = io->grad_viscosity;
+    LAUNCH_PASS(k_wadj_force<kReplayCap>, nb, s, fa);
+
+    WadjDensityArgs da;
+    da.c = make_common(g, w, pos, h, kListUse);
+    da.wvol = w->wvol;
+    da.rhobar = rhobar;
+    da.xbar_s = xbar;
+    da.vbar_s = vbar;
+    da.xbar_in = io->xbar_in;
+    da.vbar_in = io->vbar_in;
+    LAUNCH_PASS(k_wadj_density<kReplayCap>, nb, s, da);
     return FS_OK;
 }
 

@@ -198,6 +198,36 @@ typedef struct fs_wcsph_io {
 int32_t fs_wcsph_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_wcsph_params* params,
                       const fs_wcsph_io* io, fs_stream_t stream);
 
+/* WCSPH adjoint.  No counterpart in the reference, whose gradients would come
+ * from Warp's autodiff tape; this is the reverse-mode derivative of
+ * fs_wcsph_step (SolverWCSPH.step, solver_wcsph.py:431-535): density, EOS
+ * pressure, pressure and viscosity forces, symplectic Euler and the
+ * outward-damping bounds.  Surface tension is taken as 0 (its cohesion term
+ * is not differentiated); callers with non-zero surface tension must not use
+ * it. */
+typedef struct fs_wcsph_adjoint_io {
+    fs_wcsph_io forward;       /* the substep's inputs; the forward is recomputed
+                                  into its outputs (x_out, v_out, rho, pressure,
+                                  accel), as fs_wcsph_step would write them     */
+    const float* xbar_out;     /* [n*3] dL/dx_out                               */
+    const float* vbar_out;     /* [n*3] dL/dv_out                               */
+    float* xbar_in;            /* [n*3] dL/dx_in  (may alias xbar_out)          */
+    float* vbar_in;            /* [n*3] dL/dv_in  (may alias vbar_out)          */
+    float* grad_stiffness;     /* [n] += dL/d(stiffness_i) of this substep      */
+    float* grad_viscosity;     /* [n] += dL/d(viscosity_i) of this substep      */
+    void* scratch;             /* device, >= fs_wcsph_adjoint_bytes(n)          */
+} fs_wcsph_adjoint_io;
+
+size_t fs_wcsph_adjoint_bytes(int32_t capacity);
+
+/* One substep's adjoint: recompute the forward substep (grid build, density
+ * pass recording the neighbour lists, force pass), then three passes over
+ * the same lists -- reverse integration/bounds, reverse force + EOS, reverse
+ * density -- each a gather per particle with no atomics.  `work` must have
+ * list_capacity > 0. */
+int32_t fs_wcsph_adjoint_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_wcsph_params* params,
+                              const fs_wcsph_adjoint_io* io, size_t scratch_bytes, fs_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* DFSPH                                                                    */
 /* ------------------------------------------------------------------------ */

@@ -451,6 +451,169 @@ void ora_sph_apply_bounds(int n, const int* flags, const int* mask, R* x, R* v, 
 }
 
 /* ------------------------------------------------------------------------ */
+/* WCSPH adjoint (no counterpart in the reference, whose gradients would come */
+/* from Warp's tape): reverse-mode derivative of one SolverWCSPH.step        */
+/* (solver_wcsph.py:431-535) with surface_tension = 0, in gather form.       */
+/* ------------------------------------------------------------------------ */
+
+/* H(r) u, H = d(cubic_grad)/dr, on the branches of cubic_grad (zero where
+ * cubic_grad is identically zero).  cubic_grad = g(d) r with
+ *   q <= 0.5: g = 6k q(3q-2)/(d h),  g' = 18k/h^3
+ *   q >  0.5: g = -6k (1-q)^2/(d h), g' = 6k/h (2(1-q)/(h d) + (1-q)^2/d^2)
+ * and H u = g u + (g'/d)(r.u) r. */
+static inline v3 cubic_hess_mul(v3 r, R r_norm, R h, v3 u)
+{
+    R k = RC(8.0) / (rpi() * (h * h * h));
+    R q = r_norm / h;
+    if (!(r_norm > RC(1.0e-5) && q <= RC(1.0))) return VZERO;
+    R g, gpd;
+    if (q <= RC(0.5)) {
+        g = RC(6.0) * k * q * (RC(3.0) * q - RC(2.0)) / (r_norm * h);
+        gpd = RC(18.0) * k / (h * h * h) / r_norm;
+    } else {
+        R t = RC(1.0) - q;
+        g = RC(-6.0) * k * t * t / (r_norm * h);
+        gpd = RC(6.0) * k / h * (RC(2.0) * t / (h * r_norm) + t * t / (r_norm * r_norm)) / r_norm;
+    }
+    return vadd(smul(g, u), smul(gpd * vdot(r, u), r));
+}
+
+/* Pass A: reverse of integrate_velocity, integrate_positions and the
+ * outward-damping apply_bounds (solver_wcsph.py:231-305).  Writes the
+ * direct parts of xbar_in / vbar_in and abar = dL/da. */
+void ora_wcsph_adj_integrate(int n, const int* flags, const int* mask, const R* x_in, const R* v_in, const R* accel,
+                             R dt, R sx, R sy, R sz, R damping, const R* xbar_out, const R* vbar_out,
+                             R* xbar_in, R* vbar_in, R* abar)
+{
+    for (int t = 0; t < n; ++t) {
+        v3 xb = vload(xbar_out, t), vb = vload(vbar_out, t);
+        if (!is_dynamic(flags[t], mask[t])) {
+            vstore(xbar_in, t, xb);
+            vstore(vbar_in, t, vb);
+            vstore(abar, t, VZERO);
+            continue;
+        }
+        v3 u = vadd(vload(v_in, t), smul(dt, vload(accel, t)));
+        v3 p = vadd(vload(x_in, t), muls(u, dt));
+        R lo[3] = {-sx, -sy, RC(0.0)}, hi[3] = {sx, sy, sz * RC(2.0)};
+        R pc[3] = {p.x, p.y, p.z}, uc[3] = {u.x, u.y, u.z};
+        R xbc[3] = {xb.x, xb.y, xb.z}, vbc[3] = {vb.x, vb.y, vb.z};
+        for (int a = 0; a < 3; ++a) {
+            if (pc[a] < lo[a]) {
+                xbc[a] = RC(0.0);
+                if (uc[a] < RC(0.0)) vbc[a] = vbc[a] * damping;
+            } else if (pc[a] > hi[a]) {
+                xbc[a] = RC(0.0);
+                if (uc[a] > RC(0.0)) vbc[a] = vbc[a] * damping;
+            }
+        }
+        v3 xbu = V(xbc[0], xbc[1], xbc[2]);
+        v3 vbu = vadd(V(vbc[0], vbc[1], vbc[2]), smul(dt, xbu));
+        vstore(xbar_in, t, xbu);
+        vstore(vbar_in, t, vbu);
+        vstore(abar, t, smul(dt, vbu));
+    }
+}
+
+/* Pass B: reverse of acceleration_wcsph (pressure + viscosity, gamma = 0) and
+ * of the EOS (solver_wcsph.py:92-106,119-228), gathered per particle k over
+ * the pairs (k,j) [k's own acceleration] and (j,k) [k as a neighbour].
+ * Accumulates into xbar_in / vbar_in and grad_stiffness / grad_viscosity,
+ * writes rhobar (dL/drho_k). */
+void ora_wcsph_adj_force(const ora_grid* g, const int* flags, const R* x, const R* v,
+                         const R* rho, const R* pressure, const R* mass, const R* rest_density, const R* stiffness,
+                         const R* exponent, const R* viscosity, R h, int clamp_negative_pressure, const R* abar,
+                         R* xbar_in, R* vbar_in, R* rhobar, R* grad_stiffness, R* grad_viscosity)
+{
+    PARFOR
+    for (int t = 0; t < g->num_points; ++t) {
+        int k = g->point_ids[t];
+        if (!is_active(flags[k])) { rhobar[k] = RC(0.0); continue; }
+        v3 xk = vload(x, k), vk = vload(v, k), ak = vload(abar, k);
+        R rho_k = rho[k], p_k = pressure[k], m_k = mass[k];
+        R P_k = p_k / rmax(rho_k * rho_k, RC(1.0e-6));
+        R w_k = m_k / rmax(rho_k, RC(1.0e-6));
+        R c_k = RC(10.0) * viscosity[k];
+        R eps2 = RC(0.01) * (h * h);
+        v3 xb = VZERO, vb = VZERO;
+        R Pb = RC(0.0), wb = RC(0.0), mub = RC(0.0);
+        FOR_CANDIDATES(g, xk, h, j, {
+            if (j == k || !is_active(flags[j])) continue;
+            v3 r = vsub(xk, vload(x, j));
+            R d = vlen(r);
+            if (!(d / h <= RC(1.0))) continue;
+            v3 G = cubic_grad(r, h);
+            v3 aj = vload(abar, j);
+            R m_j = mass[j];
+            R rho_j = rho[j];
+            R P_j = pressure[j] / rmax(rho_j * rho_j, RC(1.0e-6));
+            R w_j = m_j / rmax(rho_j, RC(1.0e-6));
+            R c_j = RC(10.0) * viscosity[j];
+            R den = d * d + eps2;
+            v3 dv = vsub(vk, vload(v, j));
+            R s = vdot(dv, r) / den;
+            R Gak = vdot(G, ak), Gaj = vdot(G, aj);
+            v3 Hak = cubic_hess_mul(r, d, h, ak), Haj = cubic_hess_mul(r, d, h, aj);
+            v3 ds = vdivs(vsub(dv, smul(RC(2.0) * s, r)), den);
+            /* pressure: f(k,j) = -m_j (P_k+P_j) G,  f(j,k) = +m_k (P_j+P_k) G */
+            xb = vadd(xb, smul(-m_j * (P_k + P_j), Hak));
+            xb = vadd(xb, smul(m_k * (P_j + P_k), Haj));
+            Pb += -m_j * Gak + m_k * Gaj;
+            /* viscosity: f(k,j) = c_k w_j s G,  f(j,k) = -c_j w_k s G */
+            R ck_wj = c_k * w_j, cj_wk = c_j * w_k;
+            xb = vadd(xb, smul(ck_wj, vadd(smul(Gak, ds), smul(s, Hak))));
+            xb = vsub(xb, smul(cj_wk, vadd(smul(Gaj, ds), smul(s, Haj))));
+            vb = vadd(vb, smul((ck_wj * Gak - cj_wk * Gaj) / den, r));
+            mub += RC(10.0) * w_j * s * Gak;
+            wb += -c_j * s * Gaj;
+        });
+        R rb = RC(0.0);
+        R pb = Pb / rmax(rho_k * rho_k, RC(1.0e-6));
+        if (rho_k * rho_k > RC(1.0e-6)) rb += Pb * (RC(-2.0) * p_k / (rho_k * rho_k * rho_k));
+        if (rho_k > RC(1.0e-6)) rb += wb * (-m_k / (rho_k * rho_k));
+        R rd = rmax(rest_density[k], RC(1.0e-6));
+        R ratio = rho_k / rd;
+        R gam = exponent[k];
+        R p_raw = stiffness[k] * (rpow(ratio, gam) - RC(1.0));
+        R sb = RC(0.0);
+        if (!clamp_negative_pressure || p_raw > RC(0.0)) {
+            rb += pb * stiffness[k] * gam * rpow(ratio, gam - RC(1.0)) / rd;
+            sb = pb * (rpow(ratio, gam) - RC(1.0));
+        }
+        rhobar[k] = rb;
+        vstore(xbar_in, k, vadd(vload(xbar_in, k), xb));
+        vstore(vbar_in, k, vadd(vload(vbar_in, k), vb));
+        grad_stiffness[k] += sb;
+        grad_viscosity[k] += mub;
+    }
+}
+
+/* Pass C: reverse of compute_density_pressure's density sum
+ * (solver_wcsph.py:119-158): xbar_k += sum_j (rhobar_k wv_j + rhobar_j wv_k) G. */
+void ora_wcsph_adj_density(const ora_grid* g, const int* flags, const R* x, const R* rest_density,
+                           const R* volume, R h, const R* rhobar, R* xbar_in)
+{
+    PARFOR
+    for (int t = 0; t < g->num_points; ++t) {
+        int k = g->point_ids[t];
+        if (!is_active(flags[k])) continue;
+        v3 xk = vload(x, k);
+        R wv_k = rest_density[k] * volume[k];
+        R rb_k = rhobar[k];
+        v3 xb = VZERO;
+        FOR_CANDIDATES(g, xk, h, j, {
+            if (j == k || !is_active(flags[j])) continue;
+            v3 r = vsub(xk, vload(x, j));
+            R d = vlen(r);
+            if (!(d / h <= RC(1.0))) continue;
+            R wv_j = rest_density[j] * volume[j];
+            xb = vadd(xb, smul(rb_k * wv_j + rhobar[j] * wv_k, cubic_grad(r, h)));
+        });
+        vstore(xbar_in, k, vadd(vload(xbar_in, k), xb));
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* DFSPH  (solver/sph/solver_dfsph.py)                                      */
 /* ------------------------------------------------------------------------ */
 

@@ -330,6 +330,76 @@ def wcsph_step(model: OModel, state_in: OState, cfg: WcsphConfig, dt: float, pre
     return out, {"accel": accel, "grid": grid, "volume": vol}
 
 
+def wcsph_adjoint_step(model: OModel, state_in: OState, cfg: WcsphConfig, dt: float, xbar_out, vbar_out,
+                       precision="f32", round_mode=0):
+    """Reverse-mode derivative of one ``wcsph_step`` (surface tension must be
+    0).  Recomputes the forward substep, then runs the three adjoint passes of
+    ``fs_oracle.c``.  Returns (xbar_in, vbar_in, grad_stiffness[n],
+    grad_viscosity[n]) -- dL/d(per-particle stiffness, viscosity) of this
+    substep."""
+    c = _Ctx(model, precision, round_mode)
+    m, lib, n = c.m, c.lib, c.m.n
+    if np.any(m.surface_tension != 0):
+        raise ValueError("the WCSPH adjoint covers surface_tension = 0 only")
+    x_in = np.ascontiguousarray(state_in.x, c.dt_np)
+    v_in = np.ascontiguousarray(state_in.v, c.dt_np)
+    out, ex = wcsph_step(model, state_in, cfg, dt, precision, round_mode)
+    xbo = np.ascontiguousarray(xbar_out, c.dt_np)
+    vbo = np.ascontiguousarray(vbar_out, c.dt_np)
+    xb = np.zeros((n, 3), c.dt_np)
+    vb = np.zeros((n, 3), c.dt_np)
+    ab = np.zeros((n, 3), c.dt_np)
+    rb = np.zeros(n, c.dt_np)
+    gs = np.zeros(n, c.dt_np)
+    gv = np.zeros(n, c.dt_np)
+    if n == 0:
+        return xb, vb, gs, gv
+    h = smoothing_length(cfg)
+    g = ex["grid"].struct()
+    lib.ora_wcsph_adj_integrate(ctypes.c_int(n), _p(m.flags), _p(m.mask), _p(x_in), _p(v_in), _p(ex["accel"]),
+                                c.r(dt), c.r(cfg.bound_width), c.r(cfg.bound_height), c.r(cfg.bound_length),
+                                c.r(cfg.boundary_damping), _p(xbo), _p(vbo), _p(xb), _p(vb), _p(ab))
+    lib.ora_wcsph_adj_force(ctypes.byref(g), _p(m.flags), _p(x_in), _p(v_in), _p(out.sph["rho"]),
+                            _p(out.sph["pressure"]), _p(m.mass), _p(m.rest_density), _p(m.stiffness),
+                            _p(m.exponent), _p(m.viscosity), c.r(h), ctypes.c_int(int(cfg.clamp_negative_pressure)),
+                            _p(ab), _p(xb), _p(vb), _p(rb), _p(gs), _p(gv))
+    lib.ora_wcsph_adj_density(ctypes.byref(g), _p(m.flags), _p(x_in), _p(m.rest_density), _p(ex["volume"]), c.r(h),
+                              _p(rb), _p(xb))
+    return xb, vb, gs, gv
+
+
+def wcsph_rollout(model: OModel, x0, v0, cfg: WcsphConfig, dt: float, steps: int, precision="f32"):
+    """``steps`` WCSPH substeps from (x0, v0); returns the list of states
+    [(x_0, v_0), ..., (x_steps, v_steps)]."""
+    st = OState(np.array(x0), np.array(v0), {})
+    states = [(st.x, st.v)]
+    for _ in range(steps):
+        st, _ = wcsph_step(model, st, cfg, dt, precision)
+        states.append((st.x, st.v))
+    return states
+
+
+def wcsph_final_state(model: OModel, x0, v0, cfg: WcsphConfig, dt: float, steps: int, precision="f64"):
+    """(x, v) after ``steps`` WCSPH substeps, keeping no intermediate states."""
+    st = OState(np.array(x0), np.array(v0), {})
+    for _ in range(steps):
+        st, _ = wcsph_step(model, st, cfg, dt, precision)
+    return st.x, st.v
+
+
+def wcsph_rollout_adjoint(model: OModel, states, cfg: WcsphConfig, dt: float, xbar, vbar, precision="f32"):
+    """Reverse sweep over a stored rollout: from dL/dx_T, dL/dv_T return
+    (dL/dx_0, dL/dv_0, dL/dstiffness[n], dL/dviscosity[n])."""
+    gs = np.zeros(model.n)
+    gv = np.zeros(model.n)
+    xb, vb = np.array(xbar, np.float64), np.array(vbar, np.float64)
+    for x, v in reversed(states[:-1]):
+        xb, vb, s, u = wcsph_adjoint_step(model, OState(x, v, {}), cfg, dt, xb, vb, precision)
+        gs += s
+        gv += u
+    return xb, vb, gs, gv
+
+
 def _dfsph_residual(values: np.ndarray, dynamic_mask: np.ndarray) -> float:
     """_density_residual / _divergence_residual  solver_dfsph.py:579-596."""
     if not np.any(dynamic_mask):
