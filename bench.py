@@ -62,6 +62,9 @@ def parse():
                                                 "microbenchmark (e.g. 4000000,16000000)")
     ap.add_argument("--batched", type=int, default=0, help="run only the batched form over this many scenes "
                                                            "(64 for the batched workload)")
+    ap.add_argument("--adjoint", action="store_true", help="run only the differentiable WCSPH form (50,540 "
+                                                            "particles, 200 substeps) with the FP64 FD check")
+    ap.add_argument("--no-fd", action="store_true", help="skip the FD check of --adjoint")
     return ap.parse_args()
 
 
@@ -293,6 +296,145 @@ def batched(num_scenes, steps, warmup, chunk=10):
     return out
 
 
+def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_scale=0.05):
+    """Differentiable WCSPH: the 50,540-particle column (x in [-1.2,-0.6],
+    y in [-0.28,0.28], z in [0,0.6], jitter 0.01 s), seeded random initial
+    velocities (``v_scale`` m/s rms per axis), 200 substeps at dt = 0.25 h/c0,
+    loss |centroid - c*|^2 + (P95(x) - x*)^2 with seeded targets.  Times the
+    plain rollout and the forward + adjoint rollout (checkpoints every 10).
+
+    With ``fd``, on the host cores: (a) the FP64 oracle's adjoint against FP64
+    central differences (steps 1e-4 and 1e-5 relative) of d/dstiffness,
+    d/dviscosity and a random direction of d/dv0; (b) the GPU gradients against
+    the FP32 oracle's adjoint of the same FP32 rollout; (c) the GPU gradients
+    against the FP64 differences directly."""
+    import numpy as np
+    import torch
+
+    from fluid_simulation_amd.adjoint import DifferentiableWCSPH, centroid_p95_loss, column_scene, dt_for
+
+    B0, NU0 = 250000.0, 1.0e-6
+    scene = column_scene(target)
+    rng = np.random.default_rng(4)
+    v0n = (rng.normal(size=(scene.count, 3)) * v_scale).astype(np.float32)
+    x0n = scene.positions
+    ctarget = x0n.mean(axis=0).astype(np.float64) + rng.normal(scale=0.02, size=3)
+    xtarget = float(np.quantile(x0n[:, 0], 0.95)) + float(rng.normal(scale=0.02))
+    dt = dt_for(scene.spacing)
+    sim = DifferentiableWCSPH(x0n, scene.mass, scene.radius, scene.spacing, dt=dt, steps=substeps,
+                              checkpoint_every=checkpoint_every)
+    x0 = torch.from_numpy(x0n).cuda()
+    v0d = torch.from_numpy(v0n).cuda()
+    ct, xt = torch.from_numpy(ctarget).cuda(), torch.tensor(xtarget, dtype=torch.float64).cuda()
+
+    def run(grad):
+        v0 = v0d.clone().requires_grad_(grad)
+        B = torch.tensor(B0, dtype=torch.float64, device="cuda", requires_grad=grad)
+        nu = torch.tensor(NU0, dtype=torch.float64, device="cuda", requires_grad=grad)
+        xT, _ = sim.rollout(x0, v0, B, nu)
+        loss = centroid_p95_loss(xT.double(), ct, xt)
+        if grad:
+            loss.backward()
+            return loss.item(), float(B.grad), float(nu.grad), v0.grad.double().cpu().numpy()
+        return loss.item(), None, None, None
+
+    run(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        with torch.no_grad():
+            run(False)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    run(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss, gB, gnu, gv = run(True)
+    torch.cuda.synchronize()
+    t_both = (time.perf_counter() - t0) / reps
+    out = {"particles": scene.count, "substeps": substeps, "checkpoint_every": checkpoint_every, "dt": dt,
+           "stiffness": B0, "viscosity": NU0, "v0_rms": v_scale, "loss": loss, "dL_dstiffness": gB,
+           "dL_dviscosity": gnu, "ms_forward": t_fwd * 1e3, "ms_forward_backward": t_both * 1e3,
+           "backward_over_forward": (t_both - t_fwd) / t_fwd,
+           "particle_substeps_per_s_forward": scene.count * substeps / t_fwd,
+           "particle_substeps_per_s_forward_backward": scene.count * substeps / t_both}
+    del sim
+    torch.cuda.empty_cache()
+    if fd:
+        from oracle import oracle
+
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        cfg = oracle.WcsphConfig(particle_radius=scene.radius, particle_length=scene.spacing,
+                                 smoothing_length_coff=2.0, bound_width=1.2, bound_height=0.28, bound_length=0.6,
+                                 boundary_damping=0.0)
+        dv = np.random.default_rng(12).normal(size=v0n.shape)
+        rel = lambda a, b: abs(a - b) / max(abs(b), 1e-300)
+        t0 = time.perf_counter()
+
+        def model(prec, B=B0, nu=NU0):
+            return oracle.uniform_model(scene.count, np.float32(scene.mass), stiffness=B, exponent=7.0,
+                                        viscosity=nu, dtype=prec)
+
+        def loss_and_grad(xT):
+            n = xT.shape[0]
+            c = xT.mean(axis=0)
+            g = np.tile(2.0 * (c - ctarget) / n, (n, 1))
+            q = 0.95 * (n - 1)
+            kq = int(np.floor(q))
+            fr = q - kq
+            order = np.argsort(xT[:, 0], kind="stable")
+            p95 = np.quantile(xT[:, 0], 0.95)
+            g[order[kq], 0] += 2.0 * (p95 - xtarget) * (1.0 - fr)
+            g[order[kq + 1], 0] += 2.0 * (p95 - xtarget) * fr
+            return float(((c - ctarget) ** 2).sum() + (p95 - xtarget) ** 2), g
+
+        def oracle_adjoint(prec):
+            npd = np.float64 if prec == "f64" else np.float32
+            states = oracle.wcsph_rollout(model(npd), x0n.astype(npd), v0n.astype(npd), cfg, dt, substeps, prec)
+            _, g = loss_and_grad(states[-1][0].astype(np.float64))
+            _, vb, gs, gvv = oracle.wcsph_rollout_adjoint(model(npd), states, cfg, dt, g, np.zeros_like(g), prec)
+            return float(gs.sum()), float(gvv.sum()), np.asarray(vb, np.float64)
+
+        a64 = oracle_adjoint("f64")
+        print(f"# adjoint check: fp64 oracle adjoint done ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
+              flush=True)
+
+        def loss64(B=B0, nu=NU0, v=v0n):
+            xT, _ = oracle.wcsph_final_state(model(np.float64, B, nu), x0n.astype(np.float64),
+                                             np.asarray(v, np.float64), cfg, dt, substeps, "f64")
+            return loss_and_grad(xT)[0]
+
+        fds = {}
+        rms = float(np.sqrt((v0n.astype(np.float64) ** 2).mean()))
+        for e in (1e-4, 1e-5):
+            fds[e] = ((loss64(B=B0 * (1 + e)) - loss64(B=B0 * (1 - e))) / (2 * e * B0),
+                      (loss64(nu=NU0 * (1 + e)) - loss64(nu=NU0 * (1 - e))) / (2 * e * NU0),
+                      (loss64(v=v0n + e * rms * dv) - loss64(v=v0n - e * rms * dv)) / (2 * e * rms))
+            print(f"# adjoint check: fp64 differences, step {e:g} done ({time.perf_counter() - t0:.0f} s)",
+                  file=sys.stderr, flush=True)
+        a32 = oracle_adjoint("f32")
+        print(f"# adjoint check: fp32 oracle adjoint done ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
+              flush=True)
+        names = ("stiffness", "viscosity", "v0_direction")
+        ad64 = (a64[0], a64[1], float((a64[2] * dv).sum()))
+        gpu = (gB, gnu, float((gv * dv).sum()))
+        check = {"step_relative": list(fds), "host_threads": threads,
+                 "fp64_oracle_adjoint_vs_fp64_fd": {n: [ad64[k]] + [fds[e][k] for e in fds] +
+                                                    [min(rel(ad64[k], fds[e][k]) for e in fds)]
+                                                    for k, n in enumerate(names)},
+                 "gpu_vs_fp32_oracle_adjoint": {"stiffness": rel(gB, a32[0]), "viscosity": rel(gnu, a32[1]),
+                                                "v0_max_abs_over_max": float(np.abs(gv - a32[2]).max() /
+                                                                             np.abs(a32[2]).max())},
+                 "gpu_vs_fp64_fd": {n: [gpu[k]] + [fds[e][k] for e in fds] +
+                                    [min(rel(gpu[k], fds[e][k]) for e in fds)] for k, n in enumerate(names)},
+                 "format": "[gradient, fd step 1e-4, fd step 1e-5, smaller relative error]",
+                 "seconds": time.perf_counter() - t0}
+        out["fd_check"] = check
+    return {"adjoint": out}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -300,6 +442,9 @@ def main():
         return
     if args.micro:
         micro([int(x) for x in args.micro.split(",")], args.steps)
+        return
+    if args.adjoint:
+        print(json.dumps(adjoint(fd=not args.no_fd)), flush=True)
         return
     if args.batched:
         print(json.dumps(batched(args.batched, args.steps, args.warmup)), flush=True)

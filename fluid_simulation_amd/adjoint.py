@@ -53,13 +53,17 @@ class ColumnScene:
         return 1000.0 * self.spacing ** 3
 
 
-def column_scene(target_count: int, jitter: float = 0.01, seed: int = 0) -> ColumnScene:
+def column_scene(target_count: int, jitter: float = 0.01, seed: int = 0, margin: float = 0.0) -> ColumnScene:
     """The column x in [-1.2,-0.6], y in [-0.28,0.28], z in [0,0.6] as a
     lattice of about ``target_count`` points: s0 = (V/N)^(1/3), n_a =
     round(L_a/s0) per axis, s = min(L_a/n_a), the lattice centred in the
     column, then seeded uniform jitter of +-jitter*s (float32).  10k -> 10,164
-    points (22 x 21 x 22); 50k -> 50,540 (38 x 35 x 38)."""
-    lo, hi = np.asarray(COLUMN_LO), np.asarray(COLUMN_HI)
+    points (22 x 21 x 22); 50k -> 50,540 (38 x 35 x 38).  ``margin`` keeps
+    the column that far from the tank walls it touches (x = -1.2, y = +-0.28,
+    z = 0), as the reference's wave-tank spawn does (scene.py:44-75)."""
+    lo, hi = np.asarray(COLUMN_LO, np.float64), np.asarray(COLUMN_HI, np.float64)
+    lo = lo + margin
+    hi[1] -= margin
     ext = hi - lo
     s0 = (float(np.prod(ext)) / target_count) ** (1.0 / 3.0)
     res = tuple(max(1, int(round(e / s0))) for e in ext)
