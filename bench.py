@@ -65,6 +65,7 @@ def parse():
     ap.add_argument("--adjoint", action="store_true", help="run only the differentiable WCSPH form (50,540 "
                                                             "particles, 200 substeps) with the FP64 FD check")
     ap.add_argument("--no-fd", action="store_true", help="skip the FD check of --adjoint")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants (no autotuning)")
     return ap.parse_args()
 
 
@@ -74,7 +75,7 @@ def driver_dt(sim):
     return min(sim.frame_dt / sim.substeps, sim.cfl_limit(*sim._cfl_maxima()))
 
 
-def bench_solver(method, scene, steps, warmup):
+def bench_solver(method, scene, steps, warmup, tune=True):
     import torch
 
     from fluid_simulation_amd import _lib
@@ -84,6 +85,16 @@ def bench_solver(method, scene, steps, warmup):
     for _ in range(warmup):
         sim._substep(driver_dt(sim))
     dt = driver_dt(sim)
+    tuned = None
+    if method == "wcsph" and tune:
+        from fluid_simulation_amd.autotune import tune as tune_passes
+
+        # variants give identical results: time them on scratch output state
+        s_scratch = sim.model.state()
+        tuned = tune_passes(lambda: sim.solver.step(sim.state_0, s_scratch, None, None, dt),
+                            ("wcsph_density", "wcsph_force"))
+        print("# wcsph autotune: " + ", ".join(f"{k} -> block {v['block']} depth {v['list_depth']}"
+                                               for k, v in tuned.items()), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -108,6 +119,9 @@ def bench_solver(method, scene, steps, warmup):
     rate = n * steps / elapsed
     out = {"particle_substeps_per_s": rate, "ms_per_substep": 1e3 * elapsed / steps, "dt": dt,
            "kernels_ms_per_substep": {k: v[1] / steps for k, v in sorted(prof.items())}}
+    if tuned:
+        out["autotune"] = {k: {"block": v["block"], "list_depth": v["list_depth"],
+                               "us_per_launch_by_variant": v["us_per_launch"]} for k, v in tuned.items()}
     if method == "dfsph":
         out["iterations_last_substep"] = iters
         b = dfsph_bytes(iters["density_iterations"], iters["density_corrections"], iters["divergence_iterations"],
@@ -223,6 +237,9 @@ def batched(num_scenes, steps, warmup, chunk=10):
     for _ in range(warmup):
         batch.substep()
     batch.check_status()
+    tuned = batch.autotune(use_cache=False)
+    print("# batched autotune: " + ", ".join(f"{k} -> block {v['block']} depth {v['list_depth']}"
+                                             for k, v in tuned.items()), file=sys.stderr, flush=True)
     batch.remaining.fill_(1.0e9)          # no scene reaches the end of its time inside the timed region
     torch.cuda.synchronize()
 
@@ -290,7 +307,9 @@ def batched(num_scenes, steps, warmup, chunk=10):
                        "particle_substeps_per_s_graph": total / graphed,
                        "particle_substeps_per_s_sequential_eager": total / seq_eager,
                        "speedup_vs_sequential_eager": seq_eager / graphed,
-                       "speedup_vs_sequential_graph": seq_graph / graphed}}
+                       "speedup_vs_sequential_graph": seq_graph / graphed,
+                       "autotune": {k: {"block": v["block"], "list_depth": v["list_depth"],
+                                        "us_per_launch_by_variant": v["us_per_launch"]} for k, v in tuned.items()}}}
     del graph, sims, batch
     torch.cuda.empty_cache()
     return out
@@ -463,7 +482,7 @@ def main():
     for method in [s for s in args.solvers.split(",") if s]:
         print(f"# bench {method}: n={scene.count} steps={args.steps} warmup={args.warmup}", file=sys.stderr,
               flush=True)
-        res, prof = bench_solver(method, scene, args.steps, args.warmup)
+        res, prof = bench_solver(method, scene, args.steps, args.warmup, tune=not args.no_tune)
         results[method] = res
         if method == "wcsph":
             prof_wcsph = prof

@@ -281,6 +281,29 @@ class BatchedWCSPH:
         self.cfl()
         self.advance()
 
+    def autotune(self, use_cache: bool = True) -> dict:
+        """Pick the fastest launch variant of the batched density and force
+        passes on this batch (results are identical under every variant);
+        the simulation state is restored afterwards."""
+        from .autotune import autotune
+
+        cur, total = self.cur, self.total_substeps
+        x, v = self.x[cur].clone(), self.v[cur].clone()
+        rho, p, a = self.rho.clone(), self.pressure.clone(), self.accel.clone()
+        dts = self.dts()
+        if not np.any(dts > 0):
+            self.set_dt(np.full(self.num_scenes, 1.0e-5, np.float32))
+        res = autotune(self.advance, ("batch_density", "batch_force"), f"batched-{self.num_scenes}-{self.n}",
+                       use_cache=use_cache)
+        self.cur, self.total_substeps = cur, total
+        self.x[cur].copy_(x)
+        self.v[cur].copy_(v)
+        self.rho.copy_(rho)
+        self.pressure.copy_(p)
+        self.accel.copy_(a)
+        self.set_dt(dts)
+        return res
+
     def check_status(self) -> None:
         """Raise the reference driver's errors for what the device CFL flagged
         (simulation.py:270-276, :305-307); synchronises."""

@@ -17,6 +17,9 @@
 // recomputed from it), so every pass after the positions change scans again.
 #include "fs_internal.h"
 
+#include <atomic>
+#include <mutex>
+
 namespace fs {
 
 constexpr int kPassBlock = 256;
@@ -47,6 +50,8 @@ struct PassCommon {
 };
 
 __device__ __forceinline__ int pass_slot() { return xcd_tile() * kPassBlock + threadIdx.x; }
+template <int BLOCK>
+__device__ __forceinline__ int pass_slot_b() { return xcd_tile() * BLOCK + threadIdx.x; }
 
 // Per-lane view of a batch scene: its bucket table, cell width and radius.
 __device__ __forceinline__ void scene_view(PassCommon& c, const fs_scene& sc)
@@ -69,6 +74,10 @@ struct NoCount {
 #define NEIGHBORS(c, s, px, py, pz, count, body)                                                           \
     for_each_neighbor<kPassBlock, CAP>((c).g, (c).pos, (s), (px), (py), (pz), (c).h, (c).cut2, lds, (c).L, \
                                             count, body)
+// the same for kernels templated on their block size BLOCK
+#define NEIGHBORS_B(c, s, px, py, pz, count, body)                                                    \
+    for_each_neighbor<BLOCK, CAP>((c).g, (c).pos, (s), (px), (py), (pz), (c).h, (c).cut2, lds, (c).L, \
+                                  count, body)
 
 // ---------------------------------------------------------------------------
 // DFSPH residual: each block of the rate pass stores its partial sums; a
@@ -157,11 +166,11 @@ struct WcsphDensityArgs {
 };
 
 // compute_density_pressure  solver_wcsph.py:119-158
-template <int CAP, bool BATCH>
-__global__ __launch_bounds__(kPassBlock) void k_wcsph_density(WcsphDensityArgs a)
+template <int BLOCK, int CAP, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_wcsph_density(WcsphDensityArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
-    const int s = pass_slot();
+    __shared__ int lds[BLOCK * CAP];
+    const int s = pass_slot_b<BLOCK>();
     if (s >= a.c.n) return;
     PassCommon c = a.c;
     int clamp = a.clamp;
@@ -182,7 +191,7 @@ __global__ __launch_bounds__(kPassBlock) void k_wcsph_density(WcsphDensityArgs a
     }
     const float h = c.h;
     float rho_temp = 0.0f;
-    NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+    NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
         if (!(c.sflags[k] & kActive)) return;
         v3 dist = vsub(xyz(xi), xyz(c.pos[k]));
         float mass_nei = a.wvol[k];
@@ -225,11 +234,11 @@ struct WcsphForceArgs {
 // p/max(rho^2,1e-6), m/max(rho,1e-6) precomputed -- the same float operations
 // on the same operands.  A zero cohesion (gamma) or viscosity (mu) coefficient
 // contributes an exact zero, so its term is skipped.
-template <int CAP, bool BATCH>
-__global__ __launch_bounds__(kPassBlock) void k_wcsph_force(WcsphForceArgs a)
+template <int BLOCK, int CAP, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_wcsph_force(WcsphForceArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
-    const int s = pass_slot();
+    __shared__ int lds[BLOCK * CAP];
+    const int s = pass_slot_b<BLOCK>();
     if (s >= a.c.n) return;
     PassCommon c = a.c;
     float pr = a.particle_radius, dt = a.dt, dmp = a.damping;
@@ -261,7 +270,7 @@ __global__ __launch_bounds__(kPassBlock) void k_wcsph_force(WcsphForceArgs a)
     const float visc_den0 = 0.01f * (h * h);
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
-    NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+    NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
         if (k == s || !(c.sflags[k] & kActive)) return;
         const float4 xj = c.pos[k];
         const float4 sj = a.sca[k];
@@ -1101,6 +1110,60 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_xsph(PbfXsphArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// launch variants of the tunable WCSPH passes: workgroup size and the depth of
+// the per-lane LDS survivor list (which bounds occupancy through LDS).  Every
+// variant visits the same candidates in the same order, so results are
+// bit-identical; only speed differs.  The selection is process-wide.
+// ---------------------------------------------------------------------------
+
+struct Variant {
+    int block;
+    int cap;
+};
+constexpr Variant kScanVariants[] = {{256, 32}, {256, 24}, {256, 16}, {128, 32}, {128, 24}, {128, 16},
+                                     {64, 32},  {64, 24},  {64, 48}};
+constexpr Variant kReplayVariants[] = {{256, 16}, {128, 16}, {64, 16}, {512, 16}};
+constexpr int kNumScan = sizeof(kScanVariants) / sizeof(kScanVariants[0]);
+constexpr int kNumReplay = sizeof(kReplayVariants) / sizeof(kReplayVariants[0]);
+constexpr int kTunePasses = 4;   // FS_TUNE_WCSPH_DENSITY .. FS_TUNE_BATCH_FORCE
+
+static std::atomic<int> g_tune[kTunePasses];
+static std::mutex g_tune_mutex;
+
+static int tune_count(int pass) { return (pass == 0 || pass == 2) ? kNumScan : kNumReplay; }
+static int tuned(int pass) { return g_tune[pass].load(std::memory_order_relaxed); }
+
+template <bool BATCH, int I = 0>
+static int32_t launch_density(int v, int n, hipStream_t s, const WcsphDensityArgs& a)
+{
+    if constexpr (I < kNumScan) {
+        if (v == I) {
+            constexpr int B = kScanVariants[I].block, C = kScanVariants[I].cap;
+            FS_LAUNCH((k_wcsph_density<B, C, BATCH>), dim3(blocks_for(n, B)), dim3(B), 0, s, a);
+            return FS_OK;
+        }
+        return launch_density<BATCH, I + 1>(v, n, s, a);
+    }
+    set_error("unknown launch variant %d", v);
+    return FS_ERR_ARG;
+}
+
+template <bool BATCH, int I = 0>
+static int32_t launch_force(int v, int n, hipStream_t s, const WcsphForceArgs& a)
+{
+    if constexpr (I < kNumReplay) {
+        if (v == I) {
+            constexpr int B = kReplayVariants[I].block, C = kReplayVariants[I].cap;
+            FS_LAUNCH((k_wcsph_force<B, C, BATCH>), dim3(blocks_for(n, B)), dim3(B), 0, s, a);
+            return FS_OK;
+        }
+        return launch_force<BATCH, I + 1>(v, n, s, a);
+    }
+    set_error("unknown launch variant %d", v);
+    return FS_ERR_ARG;
+}
+
+// ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
 
@@ -1257,7 +1320,8 @@ static int32_t wcsph_forward(fs_grid* g, fs_work* w, const fs_model* m, const fs
     da.vel = reinterpret_cast<float4*>(w->vel);
     da.scenes = nullptr;
     da.sscene = nullptr;
-    LAUNCH_PASS((k_wcsph_density<kScanCap, false>), nb, s, da);
+    rc = launch_density<false>(tuned(0), n, s, da);
+    if (rc) return rc;
 
     WcsphForceArgs fa;
     fa.c = make_common(g, w, pos, h, kListUse);
@@ -1278,8 +1342,7 @@ static int32_t wcsph_forward(fs_grid* g, fs_work* w, const fs_model* m, const fs
     fa.v_out = io->v_out;
     fa.scenes = nullptr;
     fa.sscene = nullptr;
-    LAUNCH_PASS((k_wcsph_force<kReplayCap, false>), nb, s, fa);
-    return FS_OK;
+    return launch_force<false>(tuned(1), n, s, fa);
 }
 
 static int32_t check_wcsph(const fs_grid* g, const fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
@@ -1308,6 +1371,48 @@ static size_t adjoint_scratch_bytes(size_t cap)
 }  // namespace fs
 
 extern "C" {
+
+int32_t fs_tune_variant_count(int32_t pass)
+{
+    if (pass < 0 || pass >= kTunePasses) {
+        set_error("fs_tune_variant_count: unknown pass %d", pass);
+        return FS_ERR_ARG;
+    }
+    return tune_count(pass);
+}
+
+int32_t fs_tune_describe(int32_t pass, int32_t variant, int32_t* block, int32_t* list_depth)
+{
+    if (pass < 0 || pass >= kTunePasses || variant < 0 || variant >= tune_count(pass)) {
+        set_error("fs_tune_describe: unknown pass %d / variant %d", pass, variant);
+        return FS_ERR_ARG;
+    }
+    const Variant& v = (pass == 0 || pass == 2) ? kScanVariants[variant] : kReplayVariants[variant];
+    if (block) *block = v.block;
+    if (list_depth) *list_depth = v.cap;
+    return FS_OK;
+}
+
+int32_t fs_tune_select(int32_t pass, int32_t variant)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (pass < 0 || pass >= kTunePasses || variant < 0 || variant >= tune_count(pass)) {
+        set_error("fs_tune_select: unknown pass %d / variant %d", pass, variant);
+        return FS_ERR_ARG;
+    }
+    g_tune[pass].store(variant, std::memory_order_relaxed);
+    return FS_OK;
+}
+
+int32_t fs_tune_selected(int32_t pass)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (pass < 0 || pass >= kTunePasses) {
+        set_error("fs_tune_selected: unknown pass %d", pass);
+        return FS_ERR_ARG;
+    }
+    return tuned(pass);
+}
 
 int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm, const fs_wcsph_io* io,
                       fs_stream_t stream)
@@ -1451,7 +1556,8 @@ int32_t fs_batch_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_
     da.pressure = io->pressure;
     da.sca = reinterpret_cast<float4*>(w->sca);
     da.vel = reinterpret_cast<float4*>(w->vel);
-    LAUNCH_PASS((k_wcsph_density<kScanCap, true>), nb, s, da);
+    rc = launch_density<true>(tuned(2), n, s, da);
+    if (rc) return rc;
 
     WcsphForceArgs fa;
     fa.c = make_common(g, w, pos, 1.0f, kListUse);
@@ -1472,8 +1578,7 @@ int32_t fs_batch_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_
     fa.accel = io->accel;
     fa.x_out = io->x_out;
     fa.v_out = io->v_out;
-    LAUNCH_PASS((k_wcsph_force<kReplayCap, true>), nb, s, fa);
-    return FS_OK;
+    return launch_force<true>(tuned(3), n, s, fa);
 }
 
 int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm, const fs_dfsph_io* io,
