@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fluidsim.h"
 
 namespace fs {
@@ -245,14 +247,14 @@ struct QueryRows {
     int row_base;
     int k, kend;             // current slot range
 
-    __device__ __forceinline__ void begin(const GridView& g, float px, float py, float pz, float r)
+    __device__ __forceinline__ void begin(const GridView& g, float qx, float qy, float qz, float r)
     {
-        int xs = cell_coord((px - r) * g.inv, g.round_mode);
-        int ysv = cell_coord((py - r) * g.inv, g.round_mode);
-        int zsv = cell_coord((pz - r) * g.inv, g.round_mode);
-        int xe = cell_coord((px + r) * g.inv, g.round_mode);
-        int ye = cell_coord((py + r) * g.inv, g.round_mode);
-        int ze = cell_coord((pz + r) * g.inv, g.round_mode);
+        int xs = cell_coord((qx - r) * g.inv, g.round_mode);
+        int ysv = cell_coord((qy - r) * g.inv, g.round_mode);
+        int zsv = cell_coord((qz - r) * g.inv, g.round_mode);
+        int xe = cell_coord((qx + r) * g.inv, g.round_mode);
+        int ye = cell_coord((qy + r) * g.inv, g.round_mode);
+        int ze = cell_coord((qz + r) * g.inv, g.round_mode);
         xe = min(xe, xs + g.dx - 1);
         ye = min(ye, ysv + g.dy - 1);
         ze = min(ze, zsv + g.dz - 1);
@@ -319,6 +321,21 @@ constexpr int kListBuild = 1;
 constexpr int kListUse = 2;
 constexpr int kBatch = 8;
 
+struct NoCount {
+    __device__ __forceinline__ void operator()(int) const {}
+};
+
+// gfx9 buffer resource over `count` float4s: 32-bit offsets, loads past the
+// end return zeros.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t float4_rsrc(const float4* p, int count)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, count * 16, 0x00020000);
+}
+__device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, int base_bytes, int imm)
+{
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, base_bytes, imm, 0));
+}
+
 // Visit the candidates of hash_grid_query(p, r) in Warp order, but call
 // `body(k)` (k = sorted slot) only for those with squared distance not above
 // `cut2` -- every skipped candidate has q > 1 and contributes exactly zero to
@@ -327,16 +344,24 @@ constexpr int kBatch = 8;
 // the expensive body runs on full waves.  `count(k)` is called for every
 // candidate (DFSPH's unfiltered neighbour count).  With a kListBuild list the
 // survivors (radius^2 <= L.cut2) are also recorded for later passes.
-template <int BLOCK, int CAP, class Count, class Body>
+//
+// The primary form loads through a buffer resource (32-bit offsets, an
+// immediate offset per candidate; slots past a row's end are read and
+// ignored, slots past the array read as zeros) and compacts branch-free.
+// LEAN is the fallback of the list-replaying passes: global loads clamped to
+// the row and branchy compaction, which keep that rarely taken path from
+// raising the register count (and lowering the occupancy) of its kernel.
+template <int BLOCK, int CAP, bool LEAN = false, class Count, class Body>
 __device__ __forceinline__ void scan_neighbors(const GridView& g, const float4* __restrict__ pos, int s, float px,
                                                float py, float pz, float r, float cut2, int* __restrict__ lds,
                                                const NbrList& L, Count&& count, Body&& body)
 {
     static_assert(CAP >= 2 * kBatch, "list capacity");
-    QueryRows it;
-    it.begin(g, px, py, pz, r);
+    constexpr bool kCounts = !__is_same(typename std::decay<Count>::type, NoCount);
     const bool record = L.mode == kListBuild;
     const float keep2 = record ? fmax_(L.cut2, cut2) : cut2;
+    QueryRows it;
+    it.begin(g, px, py, pz, r);
     int written = 0;
     if (it.valid()) {
         int* list = lds + threadIdx.x;
@@ -352,15 +377,30 @@ __device__ __forceinline__ void scan_neighbors(const GridView& g, const float4* 
                 }
                 const int take = rem < kBatch ? rem : kBatch;
                 float4 q[kBatch];
+                if constexpr (LEAN) {
 #pragma unroll
-                for (int u = 0; u < kBatch; ++u) q[u] = pos[k0 + (u < take ? u : take - 1)];
+                    for (int u = 0; u < kBatch; ++u) q[u] = pos[k0 + (u < take ? u : take - 1)];
 #pragma unroll
-                for (int u = 0; u < kBatch; ++u) {
-                    if (u < take) {
-                        count(k0 + u);
+                    for (int u = 0; u < kBatch; ++u) {
+                        if (u < take) {
+                            count(k0 + u);
+                            float ddx = px - q[u].x, ddy = py - q[u].y, ddz = pz - q[u].z;
+                            float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            if (!(d2 > keep2)) list[(cnt++) * BLOCK] = k0 + u;
+                        }
+                    }
+                } else {
+                    const __amdgpu_buffer_rsrc_t rs = float4_rsrc(pos, g.npts);
+                    const int off = k0 * 16;
+#pragma unroll
+                    for (int u = 0; u < kBatch; ++u) q[u] = ld_f4(rs, off, u * 16);
+#pragma unroll
+                    for (int u = 0; u < kBatch; ++u) {
+                        if (kCounts && u < take) count(k0 + u);
                         float ddx = px - q[u].x, ddy = py - q[u].y, ddz = pz - q[u].z;
                         float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        if (!(d2 > keep2)) list[(cnt++) * BLOCK] = k0 + u;
+                        list[cnt * BLOCK] = k0 + u;
+                        cnt += (u < take && !(d2 > keep2)) ? 1 : 0;
                     }
                 }
                 it.k = k0 + take;
@@ -379,35 +419,42 @@ __device__ __forceinline__ void scan_neighbors(const GridView& g, const float4* 
     if (record) L.cnt[s] = written <= L.cap ? written : L.cap + 1;
 }
 
-// Neighbour loop of one pass: replay the recorded list when there is one (and
-// it did not overflow), otherwise scan the grid.
-template <int BLOCK, int CAP, class Count, class Body>
+// Neighbour loop of one pass.  A replaying pass (REPLAY) walks the list the
+// scanning pass recorded at the same positions, and scans only when there is
+// no list or it overflowed (the lean scan); a scanning pass always scans with
+// the primary loop and records when its list is in build mode.  The role is
+// a template parameter so that each kernel carries only the loop it runs.
+template <int BLOCK, int CAP, bool REPLAY, class Count, class Body>
 __device__ __forceinline__ void for_each_neighbor(const GridView& g, const float4* __restrict__ pos, int s, float px,
                                                   float py, float pz, float r, float cut2, int* __restrict__ lds,
                                                   const NbrList& L, Count&& count, Body&& body)
 {
-    if (L.mode == kListUse) {
-        const int m = L.cnt[s];
-        if (m <= L.cap) {
-            const int* __restrict__ idx = L.idx + s;
-            int e = 0;
-            for (; e + 4 <= m; e += 4) {
-                const int k0 = idx[(e + 0) * L.n], k1 = idx[(e + 1) * L.n];
-                const int k2 = idx[(e + 2) * L.n], k3 = idx[(e + 3) * L.n];
-                body(k0);
-                body(k1);
-                body(k2);
-                body(k3);
+    if constexpr (REPLAY) {
+        if (L.mode == kListUse) {
+            const int m = L.cnt[s];
+            if (m <= L.cap) {
+                const int* __restrict__ idx = L.idx + s;
+                int e = 0;
+                for (; e + 4 <= m; e += 4) {
+                    const int k0 = idx[(e + 0) * L.n], k1 = idx[(e + 1) * L.n];
+                    const int k2 = idx[(e + 2) * L.n], k3 = idx[(e + 3) * L.n];
+                    body(k0);
+                    body(k1);
+                    body(k2);
+                    body(k3);
+                }
+                for (; e < m; ++e) body(idx[e * L.n]);
+                return;
             }
-            for (; e < m; ++e) body(idx[e * L.n]);
-            return;
         }
         NbrList none = L;
         none.mode = kListNone;
-        scan_neighbors<BLOCK, CAP>(g, pos, s, px, py, pz, r, cut2, lds, none, count, body);
-        return;
+        scan_neighbors<BLOCK, CAP, true>(g, pos, s, px, py, pz, r, cut2, lds, none, count, body);
+    } else {
+        NbrList scan = L;
+        if (scan.mode == kListUse) scan.mode = kListNone;
+        scan_neighbors<BLOCK, CAP, false>(g, pos, s, px, py, pz, r, cut2, lds, scan, count, body);
     }
-    scan_neighbors<BLOCK, CAP>(g, pos, s, px, py, pz, r, cut2, lds, L, count, body);
 }
 
 // Block index -> tile index so that each of the 8 XCDs (which receive

@@ -67,17 +67,15 @@ __device__ __forceinline__ void scene_view(PassCommon& c, const fs_scene& sc)
     c.L.cut2 = c.cut2;
 }
 
-struct NoCount {
-    __device__ __forceinline__ void operator()(int) const {}
-};
-
-#define NEIGHBORS(c, s, px, py, pz, count, body)                                                           \
-    for_each_neighbor<kPassBlock, CAP>((c).g, (c).pos, (s), (px), (py), (pz), (c).h, (c).cut2, lds, (c).L, \
-                                            count, body)
-// the same for kernels templated on their block size BLOCK
-#define NEIGHBORS_B(c, s, px, py, pz, count, body)                                                    \
-    for_each_neighbor<BLOCK, CAP>((c).g, (c).pos, (s), (px), (py), (pz), (c).h, (c).cut2, lds, (c).L, \
-                                  count, body)
+// neighbour loop of a scanning pass (SCAN_*) or of a list-replaying pass
+// (REPLAY_*); the _B forms are for kernels templated on their block size BLOCK
+#define NBR_LOOP(blk, rep, c, s, px, py, pz, count, body)                                                \
+    for_each_neighbor<blk, CAP, rep>((c).g, (c).pos, (s), (px), (py), (pz), (c).h, (c).cut2, lds, (c).L, \
+                                     count, body)
+#define SCAN_NEIGHBORS(...) NBR_LOOP(kPassBlock, false, __VA_ARGS__)
+#define REPLAY_NEIGHBORS(...) NBR_LOOP(kPassBlock, true, __VA_ARGS__)
+#define SCAN_NEIGHBORS_B(...) NBR_LOOP(BLOCK, false, __VA_ARGS__)
+#define REPLAY_NEIGHBORS_B(...) NBR_LOOP(BLOCK, true, __VA_ARGS__)
 
 // ---------------------------------------------------------------------------
 // DFSPH residual: each block of the rate pass stores its partial sums; a
@@ -191,7 +189,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcsph_density(WcsphDensityArgs a)
     }
     const float h = c.h;
     float rho_temp = 0.0f;
-    NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+    SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
         if (!(c.sflags[k] & kActive)) return;
         v3 dist = vsub(xyz(xi), xyz(c.pos[k]));
         float mass_nei = a.wvol[k];
@@ -270,7 +268,7 @@ __global__ __launch_bounds__(BLOCK) void k_wcsph_force(WcsphForceArgs a)
     const float visc_den0 = 0.01f * (h * h);
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
-    NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+    REPLAY_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
         if (k == s || !(c.sflags[k] & kActive)) return;
         const float4 xj = c.pos[k];
         const float4 sj = a.sca[k];
@@ -408,7 +406,7 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_force(WadjForceArgs a)
     const float eps2 = 0.01f * (h * h);
     v3 xb = V(0.0f, 0.0f, 0.0f), vb = V(0.0f, 0.0f, 0.0f);
     float Pb = 0.0f, wb = 0.0f, mub = 0.0f;
-    NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, [&](int j) {
+    REPLAY_NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, [&](int j) {
         if (j == s || !(c.sflags[j] & kActive)) return;
         const float4 xj4 = c.pos[j];
         const v3 r = vsub(xk, xyz(xj4));
@@ -492,7 +490,7 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_density(WadjDensityArgs a)
         const float wv_k = a.wvol[s];
         const float rb_k = a.rhobar[s];
         const float h = c.h;
-        NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, [&](int j) {
+        REPLAY_NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, [&](int j) {
             if (j == s || !(c.sflags[j] & kActive)) return;
             const v3 r = vsub(xk, xyz(c.pos[j]));
             const float d = vlen(r);
@@ -549,7 +547,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_density_factor(DfDensityAr
     float sum_grad_sq = 0.0f;
     int count = 0;
     auto counter = [&](int k) { count += (k != s && (a.c.sflags[k] & kActive)) ? 1 : 0; };
-    NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, counter, [&](int k) {
+    SCAN_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, counter, [&](int k) {
         if (!(a.c.sflags[k] & kActive)) return;
         const float4 xj = a.c.pos[k];
         v3 r = vsub(xi, xyz(xj));
@@ -610,7 +608,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureA
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
     if (gamma != 0.0f || mu != 0.0f) {
-        NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+        REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
             if (k == s || !(a.c.sflags[k] & kActive)) return;
             const float4 xj = a.c.pos[k];
             const v3 d = vsub(xi, xyz(xj));
@@ -674,7 +672,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
             const v3 vi = xyz(a.vstar_s[s]);
             const float h = a.c.h;
             float div_val = 0.0f;
-            NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+            REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
                 if (!(a.c.sflags[k] & kActive)) return;
                 const float4 xj = a.c.pos[k];
                 v3 grad_w = cubic_grad(vsub(xi, xyz(xj)), h);
@@ -738,7 +736,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_correction(DfCorrArgs a)
     const float h = a.c.h;
     const float kri = si.w;
     v3 dv = V(0.0f, 0.0f, 0.0f);
-    NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+    REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
         if (!(a.c.sflags[k] & kActive)) return;
         const float4 xj = a.c.pos[k];
         float coeff = xj.w * (kri + a.sca[k].w);
@@ -860,7 +858,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_density(PbfDensityArgs a)
     float rho_temp = 0.0f;
     if (fl & kActive) {
         const float h = c.h;
-        NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+        SCAN_NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
             if (!(c.sflags[k] & kActive)) return;
             v3 r = vsub(xi, xyz(c.pos[k]));
             rho_temp += a.wvol[k] * poly6_w(vlen(r), h);
@@ -925,7 +923,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_nonpressure(PbfNonPressureAr
             const float mu10 = 10.0f * mu;
             v3 acc = V(0.0f, 0.0f, 0.0f);
             if (gamma != 0.0f || mu != 0.0f) {
-                NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+                REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
                     if (k == s || !(a.c.sflags[k] & kActive)) return;
                     const float4 xj = a.c.pos[k];
                     const v3 d = vsub(xi, xyz(xj));
@@ -988,7 +986,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_lambda(PbfLambdaArgs a)
     v3 grad_i = V(0.0f, 0.0f, 0.0f);
     float sum_grad_sq = 0.0f;
     int count = 0;
-    NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+    SCAN_NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
         if (!(c.sflags[k] & kActive)) return;
         const float4 xj = c.pos[k];
         v3 xij = vsub(xi, xyz(xj));
@@ -1044,7 +1042,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_delta(PbfDeltaArgs a)
             const float lam_i = a.sca[s].x;
             const float h = c.h;
             v3 dp = V(0.0f, 0.0f, 0.0f);
-            NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+            REPLAY_NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
                 if (k == s || !(c.sflags[k] & kActive)) return;
                 const float4 xj = c.pos[k];
                 v3 xij = vsub(xi, xyz(xj));
@@ -1096,7 +1094,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_xsph(PbfXsphArgs a)
     }
     const float h = c.h;
     v3 corr = V(0.0f, 0.0f, 0.0f);
-    NEIGHBORS(c, s, x.x, x.y, x.z, NoCount{}, [&](int k) {
+    REPLAY_NEIGHBORS(c, s, x.x, x.y, x.z, NoCount{}, [&](int k) {
         if (k == s || !(c.sflags[k] & kActive)) return;
         const float4 xj = c.pos[k];
         const float4 vj = a.vel[k];
@@ -1120,9 +1118,10 @@ struct Variant {
     int block;
     int cap;
 };
-constexpr Variant kScanVariants[] = {{256, 32}, {256, 24}, {256, 16}, {128, 32}, {128, 24}, {128, 16},
-                                     {64, 32},  {64, 24},  {64, 48}};
-constexpr Variant kReplayVariants[] = {{256, 16}, {128, 16}, {64, 16}, {512, 16}};
+// variant 0 is the default
+constexpr Variant kScanVariants[] = {{64, 24},  {64, 32},  {64, 48},  {128, 24}, {128, 32}, {128, 16},
+                                     {256, 32}, {256, 24}, {256, 16}};
+constexpr Variant kReplayVariants[] = {{64, 16}, {128, 16}, {256, 16}, {512, 16}};
 constexpr int kNumScan = sizeof(kScanVariants) / sizeof(kScanVariants[0]);
 constexpr int kNumReplay = sizeof(kReplayVariants) / sizeof(kReplayVariants[0]);
 constexpr int kTunePasses = 4;   // FS_TUNE_WCSPH_DENSITY .. FS_TUNE_BATCH_FORCE
