@@ -49,6 +49,7 @@ class FsWcsphParams(ctypes.Structure):
     _fields_ = [
         ("smoothing_length", c_float), ("particle_radius", c_float), ("bound", c_float * 3),
         ("boundary_damping", c_float), ("dt", c_float), ("clamp_negative_pressure", c_int32),
+        ("dt_device", c_void_p),
     ]
 
 
@@ -152,8 +153,8 @@ SIGNATURES = {
                               POINTER(FsPbfIO), c_void_p]),
     "fs_batch_wcsph_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), c_void_p, c_int32,
                                       POINTER(FsBatchIO), c_void_p]),
-    "fs_batch_cfl_dt": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_double, c_double, c_double,
-                                  c_double, c_double, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fs_batch_cfl_dt": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_double, c_double,
+                                  c_double, c_double, c_double, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fs_norm_maxima": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),
     "fs_check_finite": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),
 }

@@ -251,7 +251,8 @@ class BatchedWCSPH:
         v = self.v[self.cur]
         spec = self.scenes[0].spec
         _lib.check(_lib.load().fs_batch_cfl_dt(
-            ctypes.c_void_p(self.table.data_ptr()), self.num_scenes, ctypes.c_void_p(v.data_ptr()),
+            ctypes.c_void_p(self.table.data_ptr()), self.num_scenes, int(max(s.count for s in self.scenes)),
+            ctypes.c_void_p(v.data_ptr()),
             ctypes.c_void_p(self.accel.data_ptr()), 1 if self.total_substeps else 0, m["stiffness"], m["exponent"],
             float(spec.density), m["viscosity"], self.frame_dt, self.substeps,
             ctypes.c_void_p(self.remaining.data_ptr()), ctypes.c_void_p(self._cfl_scratch.data_ptr()),

@@ -2,6 +2,8 @@
 // the non-finite state check (examples/dam_break/simulation.py:142-176).
 #include "fs_internal.h"
 
+#include <algorithm>
+
 namespace fs {
 
 constexpr int kRedBlock = 256;
@@ -162,9 +164,10 @@ int32_t fs_norm_maxima(const float* v, int32_t nv, const float* a, int32_t na, f
     return FS_OK;
 }
 
-int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, const float* v, const float* a, int32_t use_accel,
-                        double stiffness, double exponent, double rest_density, double viscosity, double frame_dt,
-                        int32_t substeps, double* remaining, uint32_t* scratch, int32_t* status, fs_stream_t stream)
+int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_particles, const float* v,
+                        const float* a, int32_t use_accel, double stiffness, double exponent, double rest_density,
+                        double viscosity, double frame_dt, int32_t substeps, double* remaining, uint32_t* scratch,
+                        int32_t* status, fs_stream_t stream)
 {
     if (!scenes || num_scenes <= 0 || num_scenes > 65535 || !v || (use_accel && !a) || !remaining || !scratch ||
         !status || substeps <= 0) {
@@ -172,7 +175,10 @@ int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, const float* v, co
         return FS_ERR_ARG;
     }
     hipStream_t s = (hipStream_t)stream;
-    FS_LAUNCH(k_scene_maxima, dim3(8, num_scenes), dim3(kRedBlock), 0, s, scenes, v, a, use_accel, scratch);
+    // blocks per scene: enough for the largest scene (about 8 particles per
+    // lane), at most 128
+    const int per_scene = max_scene_particles > 0 ? std::min(128, blocks_for(max_scene_particles, kRedBlock * 8)) : 8;
+    FS_LAUNCH(k_scene_maxima, dim3(per_scene, num_scenes), dim3(kRedBlock), 0, s, scenes, v, a, use_accel, scratch);
     FS_LAUNCH(k_scene_dt, dim3(blocks_for(num_scenes, 64)), dim3(64), 0, s, scenes, num_scenes, scratch, stiffness,
               exponent, rest_density, viscosity, frame_dt, substeps, remaining, status);
     return FS_OK;

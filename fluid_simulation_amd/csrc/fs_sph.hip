@@ -23,7 +23,8 @@
 namespace fs {
 
 constexpr int kPassBlock = 256;
-constexpr int kScanCap = 32;    // LDS survivor list of a scanning pass
+constexpr int kScanBlock = 64;  // workgroup of the untuned scanning passes (DFSPH, PBF)
+constexpr int kScanCap = 24;    // LDS survivor list of a scanning pass
 constexpr int kReplayCap = 16;  // ... of a list-replaying pass (only its overflow fallback scans)
 constexpr float kCutSlack = 1.0001f;   // candidates with d2 > h^2*slack have q > 1
 
@@ -157,26 +158,19 @@ struct WcsphDensityArgs {
     const float* __restrict__ stiffness;
     const float* __restrict__ exponent;
     int clamp;
+    const float* dt_device;   // plain pass: a device dt of 0 makes the pass idle
     float* rho;
     float* pressure;
     float4* sca;    // rho, p, p/max(rho^2,1e-6), -
     float4* vel;    // .w <- m/max(rho,1e-6)
 };
 
-// compute_density_pressure  solver_wcsph.py:119-158
-template <int BLOCK, int CAP, bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_wcsph_density(WcsphDensityArgs a)
+// compute_density_pressure  solver_wcsph.py:119-158, for lane s with the
+// pass parameters in `c`
+template <int BLOCK, int CAP>
+__device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, const PassCommon& c, int clamp, int s,
+                                                   int* lds)
 {
-    __shared__ int lds[BLOCK * CAP];
-    const int s = pass_slot_b<BLOCK>();
-    if (s >= a.c.n) return;
-    PassCommon c = a.c;
-    int clamp = a.clamp;
-    if (BATCH) {
-        const fs_scene& sc = a.scenes[a.sscene[s]];
-        scene_view(c, sc);
-        clamp = sc.clamp_negative_pressure;
-    }
     const int i = c.perm[s];
     const float4 xi = c.pos[s];
     if (!(c.sflags[s] & kActive)) {
@@ -204,6 +198,46 @@ __global__ __launch_bounds__(BLOCK) void k_wcsph_density(WcsphDensityArgs a)
     a.vel[s].w = xi.w / fmax_(rho_i, 1.0e-6f);
 }
 
+// Batched passes: slots are sorted by bucket and every scene owns a
+// consecutive bucket range, so a wave almost always holds a single scene.
+// The lanes take the wave's scenes one at a time (readfirstlane), each with
+// its scene parameters uniform, so they stay in scalar registers as the
+// plain pass's launch parameters do.  A scene with dt 0 is idle: its state,
+// density, pressure and acceleration stay as they are (neighbours never
+// cross scenes).
+#define FOR_MY_SCENE(a, s, sc, ...)                                   \
+    do {                                                              \
+        const int mine_ = (a).sscene[s];                              \
+        while (true) {                                                \
+            const int sid_ = __builtin_amdgcn_readfirstlane(mine_);   \
+            if (mine_ == sid_) {                                      \
+                const fs_scene& sc = (a).scenes[sid_];                \
+                __VA_ARGS__                                           \
+                break;                                                \
+            }                                                         \
+        }                                                             \
+    } while (0)
+
+template <int BLOCK, int CAP, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_wcsph_density(WcsphDensityArgs a)
+{
+    __shared__ int lds[BLOCK * CAP];
+    const int s = pass_slot_b<BLOCK>();
+    if (s >= a.c.n) return;
+    if constexpr (BATCH) {
+        FOR_MY_SCENE(a, s, sc, {
+            if (sc.dt != 0.0f) {
+                PassCommon c = a.c;
+                scene_view(c, sc);
+                wcsph_density_lane<BLOCK, CAP>(a, c, sc.clamp_negative_pressure, s, lds);
+            }
+        });
+    } else {
+        if (a.dt_device && *a.dt_device == 0.0f) return;
+        wcsph_density_lane<BLOCK, CAP>(a, a.c, a.clamp, s, lds);
+    }
+}
+
 struct WcsphForceArgs {
     PassCommon c;
     const fs_scene* __restrict__ scenes;
@@ -220,6 +254,7 @@ struct WcsphForceArgs {
     float dt;
     float bound[3];
     float damping;
+    const float* dt_device;   // plain pass: replaces dt when set
     float* accel;
     float* x_out;
     float* v_out;
@@ -232,25 +267,10 @@ struct WcsphForceArgs {
 // p/max(rho^2,1e-6), m/max(rho,1e-6) precomputed -- the same float operations
 // on the same operands.  A zero cohesion (gamma) or viscosity (mu) coefficient
 // contributes an exact zero, so its term is skipped.
-template <int BLOCK, int CAP, bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_wcsph_force(WcsphForceArgs a)
+template <int BLOCK, int CAP>
+__device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const PassCommon& c, float pr, float dt,
+                                                 float dmp, float sx, float sy, float sz, int s, int* lds)
 {
-    __shared__ int lds[BLOCK * CAP];
-    const int s = pass_slot_b<BLOCK>();
-    if (s >= a.c.n) return;
-    PassCommon c = a.c;
-    float pr = a.particle_radius, dt = a.dt, dmp = a.damping;
-    float sx = a.bound[0], sy = a.bound[1], sz = a.bound[2];
-    if (BATCH) {
-        const fs_scene& sc = a.scenes[a.sscene[s]];
-        scene_view(c, sc);
-        pr = sc.particle_radius;
-        dt = sc.dt;
-        dmp = sc.boundary_damping;
-        sx = sc.bound[0];
-        sy = sc.bound[1];
-        sz = sc.bound[2];
-    }
     const int i = c.perm[s];
     if (!(c.sflags[s] & kDynamic)) {
         st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
@@ -300,6 +320,38 @@ __global__ __launch_bounds__(BLOCK) void k_wcsph_force(WcsphForceArgs a)
     if (x.z > sz * 2.0f) { x.z = sz * 2.0f; if (v.z > 0.0f) v.z = v.z * dmp; }
     st3(a.v_out, i, v);
     st3(a.x_out, i, x);
+}
+
+template <int BLOCK, int CAP, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_wcsph_force(WcsphForceArgs a)
+{
+    __shared__ int lds[BLOCK * CAP];
+    const int s = pass_slot_b<BLOCK>();
+    if (s >= a.c.n) return;
+    if constexpr (BATCH) {
+        FOR_MY_SCENE(a, s, sc, {
+            if (sc.dt != 0.0f) {
+                PassCommon c = a.c;
+                scene_view(c, sc);
+                wcsph_force_lane<BLOCK, CAP>(a, c, sc.particle_radius, sc.dt, sc.boundary_damping, sc.bound[0],
+                                             sc.bound[1], sc.bound[2], s, lds);
+            } else {   // idle scene: carry the state over, keep its acceleration
+                const int i = a.c.perm[s];
+                st3(a.v_out, i, ld3(a.v_in, i));
+                st3(a.x_out, i, ld3(a.x_in, i));
+            }
+        });
+    } else {
+        const float dt = a.dt_device ? *a.dt_device : a.dt;
+        if (dt == 0.0f) {   // idle substep (device dt 0): carry the state over
+            const int i = a.c.perm[s];
+            st3(a.v_out, i, ld3(a.v_in, i));
+            st3(a.x_out, i, ld3(a.x_in, i));
+            return;
+        }
+        wcsph_force_lane<BLOCK, CAP>(a, a.c, a.particle_radius, dt, a.damping, a.bound[0], a.bound[1], a.bound[2], s,
+                                     lds);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -521,11 +573,11 @@ struct DfDensityArgs {
 };
 
 // compute_density + compute_factor_dfsph  solver_dfsph.py:106-174 (one pass)
-template <int CAP>
-__global__ __launch_bounds__(kPassBlock) void k_dfsph_density_factor(DfDensityArgs a)
+template <int BLOCK, int CAP>
+__global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
-    const int s = pass_slot();
+    __shared__ int lds[BLOCK * CAP];
+    const int s = pass_slot_b<BLOCK>();
     if (s >= a.c.n) return;
     const int i = a.c.perm[s];
     const float4 xi4 = a.c.pos[s];
@@ -547,7 +599,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_density_factor(DfDensityAr
     float sum_grad_sq = 0.0f;
     int count = 0;
     auto counter = [&](int k) { count += (k != s && (a.c.sflags[k] & kActive)) ? 1 : 0; };
-    SCAN_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, counter, [&](int k) {
+    SCAN_NEIGHBORS_B(a.c, s, xi.x, xi.y, xi.z, counter, [&](int k) {
         if (!(a.c.sflags[k] & kActive)) return;
         const float4 xj = a.c.pos[k];
         v3 r = vsub(xi, xyz(xj));
@@ -844,12 +896,12 @@ struct PbfDensityArgs {
 // compute_density (poly6)  solver_pbf.py:114-141; in the final pass also
 // update_velocity_from_positions :402-416 and, without XSPH, :939-954.
 // vel[s].w <- m/max(rho,1e-6) for the non-pressure and XSPH passes.
-template <int CAP>
-__global__ __launch_bounds__(kPassBlock) void k_pbf_density(PbfDensityArgs a)
+template <int BLOCK, int CAP>
+__global__ __launch_bounds__(BLOCK) void k_pbf_density(PbfDensityArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
+    __shared__ int lds[BLOCK * CAP];
     PassCommon c = a.c;
-    const int s = pass_slot();
+    const int s = pass_slot_b<BLOCK>();
     if (s >= c.n) return;
     const int i = c.perm[s];
     const int fl = c.sflags[s];
@@ -858,7 +910,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_density(PbfDensityArgs a)
     float rho_temp = 0.0f;
     if (fl & kActive) {
         const float h = c.h;
-        SCAN_NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+        SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
             if (!(c.sflags[k] & kActive)) return;
             v3 r = vsub(xi, xyz(c.pos[k]));
             rho_temp += a.wvol[k] * poly6_w(vlen(r), h);
@@ -964,12 +1016,12 @@ struct PbfLambdaArgs {
 };
 
 // compute_pbf_lambda  solver_pbf.py:245-302
-template <int CAP>
-__global__ __launch_bounds__(kPassBlock) void k_pbf_lambda(PbfLambdaArgs a)
+template <int BLOCK, int CAP>
+__global__ __launch_bounds__(BLOCK) void k_pbf_lambda(PbfLambdaArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
+    __shared__ int lds[BLOCK * CAP];
     PassCommon c = a.c;
-    const int s = pass_slot();
+    const int s = pass_slot_b<BLOCK>();
     if (s >= c.n) return;
     const int i = c.perm[s];
     if (!(c.sflags[s] & kDynamic)) {
@@ -986,7 +1038,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_lambda(PbfLambdaArgs a)
     v3 grad_i = V(0.0f, 0.0f, 0.0f);
     float sum_grad_sq = 0.0f;
     int count = 0;
-    SCAN_NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+    SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
         if (!(c.sflags[k] & kActive)) return;
         const float4 xj = c.pos[k];
         v3 xij = vsub(xi, xyz(xj));
@@ -1313,6 +1365,7 @@ static int32_t wcsph_forward(fs_grid* g, fs_work* w, const fs_model* m, const fs
     da.stiffness = m->stiffness;
     da.exponent = m->exponent;
     da.clamp = prm->clamp_negative_pressure;
+    da.dt_device = prm->dt_device;
     da.rho = io->rho;
     da.pressure = io->pressure;
     da.sca = reinterpret_cast<float4*>(w->sca);
@@ -1336,6 +1389,7 @@ static int32_t wcsph_forward(fs_grid* g, fs_work* w, const fs_model* m, const fs
     fa.dt = prm->dt;
     for (int d = 0; d < 3; ++d) fa.bound[d] = prm->bound[d];
     fa.damping = prm->boundary_damping;
+    fa.dt_device = prm->dt_device;
     fa.accel = io->accel;
     fa.x_out = io->x_out;
     fa.v_out = io->v_out;
@@ -1551,6 +1605,7 @@ int32_t fs_batch_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_
     da.stiffness = m->stiffness;
     da.exponent = m->exponent;
     da.clamp = 1;
+    da.dt_device = nullptr;
     da.rho = io->rho;
     da.pressure = io->pressure;
     da.sca = reinterpret_cast<float4*>(w->sca);
@@ -1574,6 +1629,7 @@ int32_t fs_batch_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_
     fa.dt = 0.0f;
     for (int d = 0; d < 3; ++d) fa.bound[d] = 0.0f;
     fa.damping = 0.0f;
+    fa.dt_device = nullptr;
     fa.accel = io->accel;
     fa.x_out = io->x_out;
     fa.v_out = io->v_out;
@@ -1624,7 +1680,7 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     dfa.neighbor_count = io->neighbor_count;
     dfa.sca = sca;
     dfa.vel = vel;
-    LAUNCH_PASS(k_dfsph_density_factor<kScanCap>, nb, s, dfa);
+    FS_LAUNCH((k_dfsph_density_factor<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, dfa);
 
     const PassCommon use = make_common(g, w, pos, h, kListUse);
     DfNonPressureArgs npa;
@@ -1691,7 +1747,7 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     dfa.c = make_common(g, w, pos, h, kListBuild);
     dfa.kappa_warm = prm->warm_start ? io->kappa_v_in : nullptr;
     dfa.kappa_commit = io->kappa_v;
-    LAUNCH_PASS(k_dfsph_density_factor<kScanCap>, nb, s, dfa);   // solver_dfsph.py:785-812
+    FS_LAUNCH((k_dfsph_density_factor<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, dfa);   // solver_dfsph.py:785-812
 
     const PassCommon use2 = make_common(g, w, pos, h, kListUse);
     ca.c = use2;
@@ -1794,7 +1850,7 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
     da.vel = vel;
     da.x_out = io->x_out;
     da.v_out = io->v_out;
-    LAUNCH_PASS(k_pbf_density<kScanCap>, nb, s, da);   // :738-751
+    FS_LAUNCH((k_pbf_density<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, da);   // :738-751
 
     PbfNonPressureArgs na;
     na.c = make_common(g, w, bufs[cur], h, kListUse);
@@ -1821,7 +1877,7 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
         la.rho = io->rho;
         la.neighbor_count = io->neighbor_count;
         la.sca = sca;
-        LAUNCH_PASS(k_pbf_lambda<kScanCap>, nb, s, la);
+        FS_LAUNCH((k_pbf_lambda<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, la);
 
         PbfDeltaArgs pa;
         pa.c = make_common(g, w, bufs[cur], h, kListUse);
@@ -1852,7 +1908,7 @@ int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_para
     da.c = make_common(g, w, bufs[cur], h, kListBuild);
     da.final_pass = 1;
     da.finalize = xsph ? 0 : 1;
-    LAUNCH_PASS(k_pbf_density<kScanCap>, nb, s, da);   // :878-917 (+ :939-954 without XSPH)
+    FS_LAUNCH((k_pbf_density<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, da);   // :878-917 (+ :939-954 without XSPH)
     if (xsph) {
         PbfXsphArgs xa;
         xa.c = make_common(g, w, bufs[cur], h, kListUse);

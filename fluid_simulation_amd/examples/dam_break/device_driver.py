@@ -1,0 +1,152 @@
+"""Device-resident dam-break frames: no host synchronisation per substep.
+
+The reference driver (``simulation.py:263-313``) reads the CFL maxima back to
+the host before every WCSPH/DFSPH substep.  Here:
+
+* WCSPH keeps the plain solver but takes its dt from device memory.  The
+  driver's CFL policy runs on the device (``fs_batch_cfl_dt`` on a one-scene
+  table, in float64 as the host formula).  It writes the substep's dt and the
+  frame time still remaining.  A chunk of substeps replays from one HIP
+  graph.  Once the frame time is used up, the rest of the chunk runs with
+  dt = 0, which leaves the state, density, pressure and acceleration as they
+  are.  The host reads one flag per chunk to learn whether the frame is
+  complete.
+* PBF has a fixed substep (frame/substeps), so a whole frame is one graph.
+* DFSPH keeps the host driver: its dt enters many of its passes as a launch
+  parameter.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ... import _lib
+from .simulation import DamBreakSimulation
+
+
+class DeviceResidentDamBreak:
+    def __init__(self, scene, method: str = "wcsph", device="cuda:0", substeps: int = 4, chunk: int = 16):
+        if method not in ("wcsph", "pbf"):
+            raise ValueError("device-resident frames cover wcsph and pbf; dfsph uses DamBreakSimulation")
+        if chunk < 2 or chunk % 2:
+            raise ValueError("chunk must be even (the substeps swap two state buffers)")
+        self.method = method
+        self.scene = scene
+        self.device = torch.device(device)
+        self.frame = 0
+        self.substeps = int(substeps)
+        self.chunk = int(chunk)
+        self.total_substeps = 0
+        self._graph = None
+        self.sim = DamBreakSimulation(scene, method, device=device, substeps=substeps)
+        self.frame_dt = self.sim.frame_dt
+        if method == "wcsph":
+            self._cfl_setup()
+
+    # ---- state views ----------------------------------------------------------
+    @property
+    def positions(self) -> torch.Tensor:
+        return self.sim.state_0.particle_q.tensor
+
+    @property
+    def velocities(self) -> torch.Tensor:
+        return self.sim.state_0.particle_qd.tensor
+
+    @property
+    def density(self) -> torch.Tensor:
+        return self.sim.state_0.sph.rho.tensor
+
+    # ---- device CFL --------------------------------------------------------------
+    def _cfl_setup(self):
+        sol = self.sim.solver
+        t = _lib.FsScene()
+        t.particle_offset = 0
+        t.particle_count = self.scene.count
+        t.smoothing_length = float(np.float32(sol.smoothing_length))
+        t.dt = 0.0
+        self._table = torch.frombuffer(bytearray(bytes(t)), dtype=torch.uint8).to(self.device)
+        off = _lib.FsScene.dt.offset
+        self.dt = self._table[off:off + 4].view(torch.float32)   # the substep's dt, on the device
+        self.remaining = torch.zeros(1, dtype=torch.float64, device=self.device)
+        self._scratch = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=self.device)
+
+    def _cfl(self):
+        sim = self.sim
+        mat = sim.material
+        _lib.check(_lib.load().fs_batch_cfl_dt(
+            ctypes.c_void_p(self._table.data_ptr()), 1, self.scene.count,
+            ctypes.c_void_p(sim.state_0.particle_qd.ptr), ctypes.c_void_p(sim.solver._particle_accel.ptr),
+            1 if sim.total_substeps else 0, float(mat["stiffness"]), float(mat["exponent"]),
+            float(self.scene.spec.density), float(mat["viscosity"]), self.frame_dt, self.substeps,
+            ctypes.c_void_p(self.remaining.data_ptr()), ctypes.c_void_p(self._scratch.data_ptr()),
+            ctypes.c_void_p(self.status.data_ptr()), ctypes.c_void_p(_lib.stream_handle(self.device))),
+            "fs_batch_cfl_dt")
+
+    def _wcsph_substep(self):
+        self._cfl()
+        self.sim._substep(self.dt)
+
+    def _wcsph_chunk(self):
+        for _ in range(self.chunk):
+            self._wcsph_substep()
+
+    def _pbf_frame(self):
+        dt = self.frame_dt / self.substeps
+        for _ in range(self.substeps):
+            self.sim._substep(dt)
+
+    # ---- capture -----------------------------------------------------------------
+    def _capture(self, work):
+        stream = torch.cuda.Stream(self.device)
+        stream.wait_stream(torch.cuda.current_stream(self.device))
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(stream):
+            with torch.cuda.graph(graph, stream=stream):
+                work()
+        torch.cuda.current_stream(self.device).wait_stream(stream)
+        return graph
+
+    # ---- stepping ----------------------------------------------------------------
+    def step(self) -> int:
+        """Advance one display frame; returns the substeps run (WCSPH: the
+        substeps replayed, including idle ones after the frame completed)."""
+        sim = self.sim
+        if self.method == "pbf":
+            if self.substeps % 2:
+                self._pbf_frame()              # odd substep counts: eager frames
+            else:
+                if self._graph is None:
+                    total = sim.total_substeps
+                    self._graph = self._capture(self._pbf_frame)
+                    sim.total_substeps = total
+                self._graph.replay()
+                sim.total_substeps += self.substeps
+            self.frame += 1
+            self.total_substeps += self.substeps
+            return self.substeps
+        self.remaining.fill_(self.frame_dt)
+        taken = 0
+        if sim.total_substeps == 0:
+            self._wcsph_substep()              # the first substep has no acceleration term yet
+            taken += 1
+        if self._graph is None:
+            total = sim.total_substeps
+            self._graph = self._capture(self._wcsph_chunk)
+            sim.total_substeps = total             # capture only recorded the chunk
+        while bool((self.remaining > 1.0e-12).item()):
+            self._graph.replay()
+            sim.total_substeps += self.chunk
+            taken += self.chunk
+            code = int(self.status.item())
+            if code == 1:
+                raise RuntimeError("Non-finite velocity or WCSPH acceleration")
+            if code == 2 or taken > 4096:
+                raise RuntimeError("CFL step cannot safely complete this frame")
+        self.frame += 1
+        sim.frame = self.frame
+        sim.sim_time = self.frame * self.frame_dt
+        self.total_substeps += taken
+        return taken

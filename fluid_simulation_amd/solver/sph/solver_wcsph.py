@@ -13,6 +13,8 @@ from __future__ import annotations
 import ctypes
 from dataclasses import dataclass
 
+import torch
+
 from ... import _lib
 from ...array import float32
 from ...newton_compat import AttributeAssignment
@@ -55,7 +57,15 @@ class SolverWCSPH(ParticleSolver):
         p.particle_radius = self._particle_radius()
         p.bound[:] = self._bounds
         p.boundary_damping = float(cfg.boundary_damping)
-        p.dt = float(dt)
+        if isinstance(dt, torch.Tensor):
+            # a device scalar (float32, one element): no host value needed, so a
+            # device-side dt policy and graph capture can drive whole frames
+            if dt.dtype != torch.float32 or dt.numel() != 1 or dt.device.type != "cuda":
+                raise ValueError("a tensor dt must be one float32 on the solver's device")
+            p.dt = 0.0
+            p.dt_device = dt.data_ptr()
+        else:
+            p.dt = float(dt)
         p.clamp_negative_pressure = 1 if cfg.clamp_negative_pressure else 0
         io = _lib.FsWcsphIO()
         io.x_in = ptr(state_in.particle_q)

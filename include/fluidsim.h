@@ -197,6 +197,11 @@ typedef struct fs_wcsph_params {
     float boundary_damping;
     float dt;
     int32_t clamp_negative_pressure;
+    /* Optional device scalar that replaces `dt` (NULL: use `dt`).  With it a
+     * substep needs no host value, so a device-side CFL (fs_batch_cfl_dt on
+     * a one-scene table) and graph capture drive whole frames.  A device dt
+     * of 0 leaves the state, rho, pressure and acceleration unchanged. */
+    const float* dt_device;
 } fs_wcsph_params;
 
 typedef struct fs_wcsph_io {
@@ -352,7 +357,9 @@ typedef struct fs_scene {
     float particle_radius;
     float bound[3];
     float boundary_damping;
-    float dt;                     /* this scene's substep; 0 freezes it     */
+    float dt;                     /* this scene's substep; 0 leaves the
+                                     scene's state, rho, pressure and
+                                     acceleration as they are               */
     int32_t clamp_negative_pressure;
     int32_t pad_;
 } fs_scene;
@@ -386,11 +393,13 @@ int32_t fs_batch_wcsph_step(fs_grid* grid, fs_work* work, const fs_model* model,
  * with vmax/amax the scene's maxima of |v| / |a| (a ignored when use_accel is
  * 0).  status[0] = 1 if a maximum is non-finite, 2 if a scene's dt falls
  * below 1e-10 (the reference's RuntimeError cases).  scratch: device uint32
- * [2*num_scenes], zero on first use and left zero. */
-int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, const float* v, const float* a, int32_t use_accel,
-                        double stiffness, double exponent, double rest_density, double viscosity, double frame_dt,
-                        int32_t substeps, double* remaining, uint32_t* scratch, int32_t* status,
-                        fs_stream_t stream);
+ * [2*num_scenes], zero on first use and left zero.  max_scene_particles (the
+ * largest particle_count in the table; the table itself is device memory)
+ * sizes the reduction. */
+int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_particles, const float* v,
+                        const float* a, int32_t use_accel, double stiffness, double exponent, double rest_density,
+                        double viscosity, double frame_dt, int32_t substeps, double* remaining, uint32_t* scratch,
+                        int32_t* status, fs_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Driver diagnostics  (examples/dam_break/simulation.py:142-176)           */
