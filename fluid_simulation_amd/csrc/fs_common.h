@@ -336,7 +336,8 @@ __device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, int base_bytes
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, base_bytes, imm, 0));
 }
 
-// Visit the candidates of hash_grid_query(p, r) in Warp order, but call
+// Visit the candidates of hash_grid_query(p, r) in Warp order (returning how
+// many there were), but call
 // `body(k)` (k = sorted slot) only for those with squared distance not above
 // `cut2` -- every skipped candidate has q > 1 and contributes exactly zero to
 // every kernel sum here.  Candidate positions are loaded kBatch at a time;
@@ -352,7 +353,7 @@ __device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, int base_bytes
 // the row and branchy compaction, which keep that rarely taken path from
 // raising the register count (and lowering the occupancy) of its kernel.
 template <int BLOCK, int CAP, bool LEAN = false, class Count, class Body>
-__device__ __forceinline__ void scan_neighbors(const GridView& g, const float4* __restrict__ pos, int s, float px,
+__device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* __restrict__ pos, int s, float px,
                                                float py, float pz, float r, float cut2, int* __restrict__ lds,
                                                const NbrList& L, Count&& count, Body&& body)
 {
@@ -363,6 +364,7 @@ __device__ __forceinline__ void scan_neighbors(const GridView& g, const float4* 
     QueryRows it;
     it.begin(g, px, py, pz, r);
     int written = 0;
+    int candidates = 0;
     if (it.valid()) {
         int* list = lds + threadIdx.x;
         bool done = false;
@@ -404,6 +406,7 @@ __device__ __forceinline__ void scan_neighbors(const GridView& g, const float4* 
                     }
                 }
                 it.k = k0 + take;
+                candidates += take;
             }
             for (int e = 0; e < cnt; ++e) {
                 const int k = list[e * BLOCK];
@@ -417,6 +420,7 @@ __device__ __forceinline__ void scan_neighbors(const GridView& g, const float4* 
         }
     }
     if (record) L.cnt[s] = written <= L.cap ? written : L.cap + 1;
+    return candidates;
 }
 
 // Neighbour loop of one pass.  A replaying pass (REPLAY) walks the list the
@@ -425,7 +429,7 @@ __device__ __forceinline__ void scan_neighbors(const GridView& g, const float4* 
 // the primary loop and records when its list is in build mode.  The role is
 // a template parameter so that each kernel carries only the loop it runs.
 template <int BLOCK, int CAP, bool REPLAY, class Count, class Body>
-__device__ __forceinline__ void for_each_neighbor(const GridView& g, const float4* __restrict__ pos, int s, float px,
+__device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4* __restrict__ pos, int s, float px,
                                                   float py, float pz, float r, float cut2, int* __restrict__ lds,
                                                   const NbrList& L, Count&& count, Body&& body)
 {
@@ -444,16 +448,16 @@ __device__ __forceinline__ void for_each_neighbor(const GridView& g, const float
                     body(k3);
                 }
                 for (; e < m; ++e) body(idx[e * L.n]);
-                return;
+                return 0;
             }
         }
         NbrList none = L;
         none.mode = kListNone;
-        scan_neighbors<BLOCK, CAP, true>(g, pos, s, px, py, pz, r, cut2, lds, none, count, body);
+        return scan_neighbors<BLOCK, CAP, true>(g, pos, s, px, py, pz, r, cut2, lds, none, count, body);
     } else {
         NbrList scan = L;
         if (scan.mode == kListUse) scan.mode = kListNone;
-        scan_neighbors<BLOCK, CAP, false>(g, pos, s, px, py, pz, r, cut2, lds, scan, count, body);
+        return scan_neighbors<BLOCK, CAP, false>(g, pos, s, px, py, pz, r, cut2, lds, scan, count, body);
     }
 }
 

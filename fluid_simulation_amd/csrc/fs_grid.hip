@@ -358,6 +358,8 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int* __r
         int fl = ga.flags[i];
         int act = fl & 1;
         ga.sflags[f] = act | ((act && ga.mask[i] == 0) ? kDynamic : 0);
+        if (ga.any_inactive && __ballot(!act) && (threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1)
+            atomicOr(ga.any_inactive, 1);
         if (ga.scene_of) ga.sscene[f] = ga.scene_of[i];
         if (ga.v) {
             v3 u = ld3(ga.v, i);

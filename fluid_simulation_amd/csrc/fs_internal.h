@@ -48,6 +48,7 @@ struct GatherOp {
     int* sflags = nullptr;         // bit0 active, bit1 dynamic
     const int* scene_of = nullptr; // optional: scene ids, gathered to sscene
     int* sscene = nullptr;
+    int* any_inactive = nullptr;   // optional: set to 1 when some point is inactive
 };
 
 // Full build: hash (+pre-op), bucket scan, scatter, stable fix-up (+gather).

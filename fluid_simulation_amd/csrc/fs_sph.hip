@@ -36,6 +36,7 @@ enum Control : int {
     kCorrRho = 3,
     kItVel = 4,
     kCorrVel = 5,
+    kAnyInactive = 6,   // set by the builds when some particle is inactive
     kControlWords = 64,
 };
 
@@ -570,6 +571,7 @@ struct DfDensityArgs {
     int* neighbor_count;
     float4* sca;   // rho, factor, kappa, kappa/max(rho,1e-6)
     float4* vel;   // .w <- m/max(rho,1e-6)
+    const int* any_inactive;   // set by the grid build when some particle is inactive
 };
 
 // compute_density + compute_factor_dfsph  solver_dfsph.py:106-174 (one pass)
@@ -597,9 +599,7 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
     float rho_temp = 0.0f;
     v3 sum_grad = V(0.0f, 0.0f, 0.0f);
     float sum_grad_sq = 0.0f;
-    int count = 0;
-    auto counter = [&](int k) { count += (k != s && (a.c.sflags[k] & kActive)) ? 1 : 0; };
-    SCAN_NEIGHBORS_B(a.c, s, xi.x, xi.y, xi.z, counter, [&](int k) {
+    auto body = [&](int k) {
         if (!(a.c.sflags[k] & kActive)) return;
         const float4 xj = a.c.pos[k];
         v3 r = vsub(xi, xyz(xj));
@@ -608,7 +608,20 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
         v3 grad = smul(xj.w, cubic_grad_r(r, rn, h));
         sum_grad = vadd(sum_grad, grad);
         sum_grad_sq += vdot(grad, grad);
-    });
+    };
+    // count = active candidates other than the particle itself, no distance
+    // test (solver_dfsph.py:150-160)
+    int count = 0;
+    if (*a.any_inactive) {
+        auto counter = [&](int k) { count += (k != s && (a.c.sflags[k] & kActive)) ? 1 : 0; };
+        SCAN_NEIGHBORS_B(a.c, s, xi.x, xi.y, xi.z, counter, body);
+    } else {
+        // every particle active: all candidates but the particle itself, which
+        // is always among its own candidates exactly once (its bucket lies in
+        // the query range and a range never repeats a bucket)
+        const int total = SCAN_NEIGHBORS_B(a.c, s, xi.x, xi.y, xi.z, NoCount{}, body);
+        count = total > 0 ? total - 1 : 0;
+    }
     const float denom = vdot(sum_grad, sum_grad) + sum_grad_sq;
     const float f = (count < a.min_neighbors || denom < 1.0e-6f) ? 0.0f : 1.0f / denom;
     a.rho[i] = rho_temp;
@@ -1665,12 +1678,15 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     PreOp pre;
     pre.x_in = io->x_in;
     pre.control_reset = w->control;
-    rc = build_grid(g, n, h, pre, gather_for(m, w, io->x_in, io->v_in, vel), s);
+    GatherOp ga1 = gather_for(m, w, io->x_in, io->v_in, vel);
+    ga1.any_inactive = w->control + kAnyInactive;
+    rc = build_grid(g, n, h, pre, ga1, s);
     if (rc) return rc;
     const float4* pos = reinterpret_cast<const float4*>(w->pos);
 
     DfDensityArgs dfa;
     dfa.c = make_common(g, w, pos, h, kListBuild);
+    dfa.any_inactive = w->control + kAnyInactive;
     dfa.wvol = w->wvol;
     dfa.min_neighbors = prm->min_neighbors_for_factor;
     dfa.kappa_warm = prm->warm_start ? io->kappa_in : nullptr;
@@ -1742,7 +1758,9 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     pre2.flags = m->flags;
     pre2.mask = m->mask;
     pre2.dt = dt;
-    rc = build_grid(g, n, h, pre2, gather_for(m, w, io->x_out, io->v_star, vstar_s), s);
+    GatherOp ga2 = gather_for(m, w, io->x_out, io->v_star, vstar_s);
+    ga2.any_inactive = w->control + kAnyInactive;   // same particles: sets the same flag again
+    rc = build_grid(g, n, h, pre2, ga2, s);
     if (rc) return rc;
     dfa.c = make_common(g, w, pos, h, kListBuild);
     dfa.kappa_warm = prm->warm_start ? io->kappa_v_in : nullptr;
