@@ -206,3 +206,36 @@ def test_bench_refuses_multi_gpu():
                          text=True, timeout=300)
     assert res.returncode == 2
     assert "one GPU" in res.stderr
+
+
+def test_tune_table_round_trip_and_cache(tmp_path, monkeypatch):
+    """The launch-variant table is host state: describe/select/selected work
+    without a GPU, unknown passes and variants are rejected, and the tuner's
+    cache is keyed by device and workload."""
+    import ctypes
+    import json
+
+    from fluid_simulation_amd import autotune as at
+
+    lib = _lib.load()
+    for name in at.PASSES:
+        vs = at.variants(name)
+        assert len(vs) >= 2 and all(b % 64 == 0 and d >= 16 for b, d in vs)
+        assert at.selected(name) == 0
+        at.select(name, len(vs) - 1)
+        assert at.selected(name) == len(vs) - 1
+        at.select(name, 0)
+    assert lib.fs_tune_select(99, 0) != 0
+    assert lib.fs_tune_select(0, 999) != 0
+    assert lib.fs_tune_describe(0, -1, None, None) != 0
+    b, d = ctypes.c_int32(), ctypes.c_int32()
+    assert lib.fs_tune_describe(0, 0, ctypes.byref(b), ctypes.byref(d)) == 0 and (b.value, d.value) == at.variants(
+        "wcsph_density")[0]
+    # cached selection is applied without timing anything
+    monkeypatch.setenv("FLUIDSIM_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(at.torch.cuda, "get_device_name", lambda *a: "dev")
+    (tmp_path / "tune.json").write_text(json.dumps({"dev|w": {"wcsph_force": {"variant": 2, "block": 256,
+                                                                             "list_depth": 16}}}))
+    res = at.autotune(lambda: (_ for _ in ()).throw(AssertionError("must not run")), ("wcsph_force",), "w")
+    assert res["wcsph_force"]["cached"] and at.selected("wcsph_force") == 2
+    at.select("wcsph_force", 0)
