@@ -18,6 +18,9 @@
 namespace fs {
 
 constexpr int kOrigin = 1 << 20;
+// Largest particle / point count: positions are float4s read through buffer
+// resources with 32-bit byte offsets (< 4 GiB).
+constexpr int kMaxParticles = FS_MAX_PARTICLES;
 constexpr float kPi = 3.14159265358979323846f;
 
 // ---------------------------------------------------------------------------
@@ -329,7 +332,9 @@ struct NoCount {
 // end return zeros.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t float4_rsrc(const float4* p, int count)
 {
-    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, count * 16, 0x00020000);
+    // num_records in bytes, unsigned 32-bit: the hosts cap particle counts
+    // at kMaxParticles so that it and every byte offset fit
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)((unsigned)count * 16u), 0x00020000);
 }
 __device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, int base_bytes, int imm)
 {
@@ -365,6 +370,8 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
     it.begin(g, px, py, pz, r);
     int written = 0;
     int candidates = 0;
+    // entry `written` of slot s; stepped by n so that cap * n may pass 2^31
+    int* __restrict__ out = L.idx + s;
     if (it.valid()) {
         int* list = lds + threadIdx.x;
         bool done = false;
@@ -393,7 +400,7 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
                     }
                 } else {
                     const __amdgpu_buffer_rsrc_t rs = float4_rsrc(pos, g.npts);
-                    const int off = k0 * 16;
+                    const int off = (int)((unsigned)k0 * 16u);
 #pragma unroll
                     for (int u = 0; u < kBatch; ++u) q[u] = ld_f4(rs, off, u * 16);
 #pragma unroll
@@ -411,7 +418,10 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
             for (int e = 0; e < cnt; ++e) {
                 const int k = list[e * BLOCK];
                 if (record) {
-                    if (written < L.cap) L.idx[written * L.n + s] = k;
+                    if (written < L.cap) {
+                        *out = k;
+                        out += L.n;
+                    }
                     ++written;
                 }
                 body(k);
@@ -437,17 +447,22 @@ __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4
         if (L.mode == kListUse) {
             const int m = L.cnt[s];
             if (m <= L.cap) {
+                // pointer stepping keeps the addressing 64-bit (cap * n may
+                // pass 2^31)
                 const int* __restrict__ idx = L.idx + s;
+                const int n = L.n;
                 int e = 0;
                 for (; e + 4 <= m; e += 4) {
-                    const int k0 = idx[(e + 0) * L.n], k1 = idx[(e + 1) * L.n];
-                    const int k2 = idx[(e + 2) * L.n], k3 = idx[(e + 3) * L.n];
+                    const int k0 = idx[0], k1 = idx[n];
+                    const int* __restrict__ idx2 = idx + 2 * (ptrdiff_t)n;
+                    const int k2 = idx2[0], k3 = idx2[n];
+                    idx = idx2 + 2 * (ptrdiff_t)n;
                     body(k0);
                     body(k1);
                     body(k2);
                     body(k3);
                 }
-                for (; e < m; ++e) body(idx[e * L.n]);
+                for (; e < m; ++e, idx += n) body(*idx);
                 return 0;
             }
         }

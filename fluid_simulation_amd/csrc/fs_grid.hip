@@ -523,6 +523,10 @@ int32_t fs_grid_init(fs_grid* g, void* workspace, size_t bytes, int32_t capacity
         set_error("fs_grid_init: invalid arguments");
         return FS_ERR_ARG;
     }
+    if (capacity > kMaxParticles) {
+        set_error("fs_grid_init: capacity %d exceeds the maximum of %d points", capacity, kMaxParticles);
+        return FS_ERR_ARG;
+    }
     if ((long long)dim_x * dim_y * dim_z >= (1LL << 30)) {
         set_error("fs_grid_init: grid dims too large");
         return FS_ERR_ARG;

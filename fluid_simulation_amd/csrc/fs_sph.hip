@@ -1317,6 +1317,10 @@ int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity
         set_error("fs_work_init: invalid arguments");
         return FS_ERR_ARG;
     }
+    if (capacity > kMaxParticles) {
+        set_error("fs_work_init: capacity %d exceeds the maximum of %d particles", capacity, kMaxParticles);
+        return FS_ERR_ARG;
+    }
     size_t need = fs_work_bytes(capacity, list_capacity);
     if (bytes < need) {
         set_error("fs_work_init: workspace %zu bytes < required %zu", bytes, need);

@@ -38,12 +38,19 @@ class ParticleSolver(SolverBase):
     _name = "Solver"
 
     #: neighbour-list entries reserved per particle (lists replayed by the
-    #: passes that follow a grid build; overflowing particles scan the grid)
+    #: passes that follow a grid build; overflowing particles scan the grid).
+    #: 96 covers the dam break's ~34 neighbours with room for compression;
+    #: denser inputs pass a larger ``neighbor_list_capacity`` (keyword-only,
+    #: after the reference's (model, config)).
     neighbor_list_capacity = 96
 
-    def __init__(self, model, config):
+    def __init__(self, model, config, *, neighbor_list_capacity: int | None = None):
         super().__init__(model=model)
         self.config = config
+        if neighbor_list_capacity is not None:
+            if int(neighbor_list_capacity) < 0:
+                raise ValueError("neighbor_list_capacity must be non-negative")
+            self.neighbor_list_capacity = int(neighbor_list_capacity)
         n = model.particle_count
         _lib.require_device(model.device)
         if n > 1:

@@ -65,8 +65,8 @@ class SolverPBF(ParticleSolver):
         register(builder, "delta_p", vec3, (0.0, 0.0, 0.0), AttributeAssignment.STATE)
         register(builder, "step_displacement", vec3, (0.0, 0.0, 0.0), AttributeAssignment.STATE)
 
-    def __init__(self, model, config: PbfConfig):
-        super().__init__(model, config)
+    def __init__(self, model, config: PbfConfig, **kwargs):
+        super().__init__(model, config, **kwargs)
         self._boundary_epsilon = float(config.boundary_epsilon)
 
     def _w_delta_q(self, smoothing_length: float) -> float:

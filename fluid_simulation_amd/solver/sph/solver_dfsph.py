@@ -53,8 +53,8 @@ class SolverDFSPH(ParticleSolver):
         register(builder, "neighbor_count", int32, 0, AttributeAssignment.STATE)
         register(builder, "v_star", vec3, (0.0, 0.0, 0.0), AttributeAssignment.STATE)
 
-    def __init__(self, model, config: SphConfig):
-        super().__init__(model, config)
+    def __init__(self, model, config: SphConfig, **kwargs):
+        super().__init__(model, config, **kwargs)
         self._iterations = torch.zeros(4, dtype=torch.int32, device=model.device)
 
     def last_iterations(self) -> dict:

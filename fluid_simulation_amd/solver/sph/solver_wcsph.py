@@ -44,8 +44,8 @@ class SolverWCSPH(ParticleSolver):
         for name in ("rho", "pressure"):
             register(builder, name, float32, 0.0, AttributeAssignment.STATE)
 
-    def __init__(self, model, config: SphConfig):
-        super().__init__(model, config)
+    def __init__(self, model, config: SphConfig, **kwargs):
+        super().__init__(model, config, **kwargs)
 
     def step(self, state_in, state_out, control, contacts, dt: float) -> None:
         del control, contacts

@@ -106,12 +106,16 @@ typedef struct fs_grid {
     uint32_t* sync;         /* [8] self-resetting counters                      */
 } fs_grid;
 
+/* Largest grid / workspace capacity (points, particles). */
+#define FS_MAX_PARTICLES ((1 << 28) - 1)
+
 /* Bytes of device workspace for a grid reserving `capacity` points. */
 size_t fs_grid_workspace_bytes(int32_t capacity, int32_t dim_x, int32_t dim_y, int32_t dim_z);
 
 /* Carve `workspace` (device, >= fs_grid_workspace_bytes) into the grid arrays
  * and zero the counters on `stream`.  Warp HashGrid(dim_x,dim_y,dim_z) +
- * reserve(capacity)  (solver_wcsph.py:398-402). */
+ * reserve(capacity)  (solver_wcsph.py:398-402).  capacity is at most
+ * FS_MAX_PARTICLES (positions are addressed with 32-bit byte offsets). */
 int32_t fs_grid_init(fs_grid* grid, void* workspace, size_t workspace_bytes, int32_t capacity, int32_t dim_x,
                      int32_t dim_y, int32_t dim_z, int32_t round_mode, fs_stream_t stream);
 
@@ -181,7 +185,8 @@ typedef struct fs_work {
 
 /* list_capacity: entries per particle of the neighbour lists recorded by the
  * first pass after each grid build and replayed by later passes over the
- * same candidates (0 disables them; every pass then scans the grid). */
+ * same candidates (0 disables them; every pass then scans the grid).  The
+ * lists take capacity * list_capacity * 4 bytes; capacity <= FS_MAX_PARTICLES. */
 size_t fs_work_bytes(int32_t capacity, int32_t list_capacity);
 int32_t fs_work_init(fs_work* work, void* workspace, size_t workspace_bytes, int32_t capacity,
                      int32_t list_capacity, fs_stream_t stream);

@@ -33,8 +33,8 @@ def build_model(method, positions, velocities, *, mass, radius, mat, flags=None,
     return model
 
 
-def make_solver(method, model, cfg):
-    return SOLVERS[method](model, cfg)
+def make_solver(method, model, cfg, **kwargs):
+    return SOLVERS[method](model, cfg, **kwargs)
 
 
 def oracle_model(O, model, method):

@@ -170,6 +170,21 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert lib.fs_work_bytes(1000, 96) > lib.fs_work_bytes(1000, 0) > 0
 
 
+def test_capacity_above_the_addressing_limit_is_rejected():
+    """Positions are read with 32-bit byte offsets: a grid or workspace for
+    more than FS_MAX_PARTICLES (2^28 - 1) points is refused before any device
+    work (the workspace pointer is never touched)."""
+    lib = _lib.load()
+    limit = (1 << 28) - 1
+    assert f"#define FS_MAX_PARTICLES ((1 << 28) - 1)" in open(HEADER).read()
+    g = _lib.FsGrid()
+    assert lib.fs_grid_init(ctypes.byref(g), ctypes.c_void_p(256), 1 << 62, limit + 1, 128, 128, 128, 0, None) != 0
+    assert b"maximum" in lib.fs_last_error()
+    w = _lib.FsWork()
+    assert lib.fs_work_init(ctypes.byref(w), ctypes.c_void_p(256), 1 << 62, limit + 1, 0, None) != 0
+    assert b"maximum" in lib.fs_last_error()
+
+
 def test_struct_layouts_agree_with_header(tmp_path):
     """Compile a C probe against include/fluidsim.h and compare sizeof/offsetof
     with the ctypes mirrors."""
