@@ -14,7 +14,24 @@ per launch (the per-particle byte model below x 401,856) over its mean launch
 duration, from HIP events on the launch stream.  ``cpu_baseline`` times the
 oracle's FP32 restatement of the same WCSPH substep on the host cores.
 
+The default run also adds, under ``extras``: the 64-scene batched substep
+(graph-replayed, eager and 64 sequential solves), the differentiable WCSPH
+rollout at 50,540 particles x 200 substeps (forward, forward + backward), and
+whole WCSPH frames with the host CFL driver beside the device-resident one.
+``roofline.traffic`` is the HBM bytes per launch of the dominant kernel, read
+from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary for the launch
+variant the autotuner selected.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--solvers wcsph,dfsph,pbf]
+       [--no-tune] [--no-extras] [--no-cpu-baseline]
+Other modes (each prints its own JSON lines and exits):
+  --micro 4000000,16000000   grid build / density / force at 4M and 16M points,
+                             uniform and clustered (median of >= 20 substeps)
+  --batched 64               the batched form alone
+  --adjoint [--no-fd]        the differentiable rollout, with an FP64 central
+                             finite-difference check unless --no-fd
+  --frames N                 N whole WCSPH frames, host and device-resident drivers
+  --copy-calibration         a 256 MiB device copy (the PMC byte-counter calibration)
 """
 from __future__ import annotations
 
@@ -22,6 +39,7 @@ import argparse
 import json
 import os
 import re
+import statistics
 import sys
 import time
 
@@ -200,7 +218,8 @@ def copy_calibration():
 
 def micro(sizes, reps):
     """Grid build, density pass and force pass at 4M / 16M points, uniform and
-    clustered (per-kernel HIP-event timing, mean over ``reps`` substeps)."""
+    clustered (per-kernel HIP-event timing; median over ``reps`` >= 20
+    substeps after a warm-up substep)."""
     import torch
 
     from fluid_simulation_amd import _lib
@@ -223,19 +242,21 @@ def micro(sizes, reps):
             solver = SolverWCSPH(model, cfg, neighbor_list_capacity=cap)
             solver.step(s0, s1, None, None, 1e-5)          # warm-up
             torch.cuda.synchronize()
-            lib.fs_profile_reset()
-            lib.fs_profile_enable(1)
-            for _ in range(reps):
+            samples = {"build": [], "density": [], "force": []}
+            for _ in range(max(20, reps)):
+                lib.fs_profile_reset()
+                lib.fs_profile_enable(1)
                 solver.step(s0, s1, None, None, 1e-5)
-            lib.fs_profile_enable(0)
-            prof = {}
-            for name, (cnt, ms) in _lib.profile_report().items():
-                key = re.sub(r"<.*>", "", name.strip("()"))
-                c0, m0 = prof.get(key, (0, 0.0))
-                prof[key] = (c0 + cnt, m0 + ms)
-            build = sum(prof[k][1] for k in ("k_hash_count", "k_bucket_scan", "k_scatter", "k_fix_gather")) / reps
-            dens = prof["k_wcsph_density"][1] / reps
-            force = prof["k_wcsph_force"][1] / reps
+                lib.fs_profile_enable(0)
+                prof = {}
+                for name, (_cnt, ms) in _lib.profile_report().items():
+                    key = re.sub(r"<.*>", "", name.strip("()"))
+                    prof[key] = prof.get(key, 0.0) + ms
+                samples["build"].append(sum(prof[k] for k in ("k_hash_count", "k_bucket_scan", "k_scatter",
+                                                               "k_fix_gather")))
+                samples["density"].append(prof["k_wcsph_density"])
+                samples["force"].append(prof["k_wcsph_force"])
+            build, dens, force = (statistics.median(samples[k]) for k in ("build", "density", "force"))
             nbr = float(counts.float().mean().item()) - 1.0
             print(json.dumps({"micro": dist, "particles": n, "h": solver.smoothing_length, "mean_neighbors": nbr,
                               "max_neighbors": int(counts.max().item()) - 1, "list_capacity": cap,
@@ -485,7 +506,7 @@ def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_
 
 def frames(particles=400000, count=3, chunk=16):
     """Whole display frames of the WCSPH dam break: the host CFL driver (one
-    readback of the maxima per substep, simulation.py:278-313) against the
+    readback of the maxima per substep, simulation.py:155-190) against the
     device-resident driver (CFL on the device, graph-replayed chunks of
     substeps, one completion check per chunk)."""
     import torch

@@ -235,5 +235,5 @@ class _Rollout(torch.autograd.Function):
 
 def dt_for(spacing: float, stiffness=250000.0, exponent=7.0, rest_density=1000.0, coff=2.0) -> float:
     """Fixed substep 0.25 h / c0 with c0 = sqrt(B gamma / rho0) (the
-    acoustic CFL term of simulation.py:289-310)."""
+    acoustic CFL term of simulation.py:166-187)."""
     return 0.25 * spacing * coff / math.sqrt(stiffness * exponent / rest_density)

@@ -5,7 +5,7 @@ a training loop over many scenes calls it once per scene.  Here every scene of
 a batch lives in one set of concatenated particle arrays and one segmented hash
 grid, so a substep of all scenes is one grid build plus the two neighbour
 passes -- six launches whatever the number of scenes -- and the per-scene CFL
-substep (``simulation.py:289-310``) is computed on the device, so a sequence
+substep (``simulation.py:166-187``) is computed on the device, so a sequence
 of substeps captures into one HIP graph with no host synchronisation.
 
 Scene ``s`` owns particles ``[offset_s, offset_s + N_s)`` and buckets
@@ -59,7 +59,7 @@ def make_batch_scenes(num_scenes: int = 64, seed: int = 3, count_range=(5_000, 2
 @dataclass
 class SceneParams:
     """Per-scene solver parameters, as the dam-break driver derives them
-    (``simulation.py:233-255``): h = 2 d, radius d/2, tank half-extents."""
+    (``simulation.py:110-132``): h = 2 d, radius d/2, tank half-extents."""
     smoothing_length: float
     particle_radius: float
     bound: tuple
@@ -240,7 +240,7 @@ class BatchedWCSPH:
 
     # ---- stepping ------------------------------------------------------------
     def begin_frame(self) -> None:
-        """Every scene has one display frame of time left (simulation.py:278-288)."""
+        """Every scene has one display frame of time left (simulation.py:155-165)."""
         self.remaining.fill_(self.frame_dt)
 
     def cfl(self) -> None:
@@ -307,7 +307,7 @@ class BatchedWCSPH:
 
     def check_status(self) -> None:
         """Raise the reference driver's errors for what the device CFL flagged
-        (simulation.py:270-276, :305-307); synchronises."""
+        (simulation.py:148-153, :182-184); synchronises."""
         code = int(self.status.item())
         if code == 1:
             raise RuntimeError("Non-finite velocity or WCSPH acceleration in a batched scene")

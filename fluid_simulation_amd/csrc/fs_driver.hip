@@ -1,5 +1,5 @@
 // fs_driver.hip -- device reductions of the dam-break driver: CFL maxima and
-// the non-finite state check (examples/dam_break/simulation.py:142-176).
+// the non-finite state check (examples/dam_break/simulation.py:19-53).
 #include "fs_internal.h"
 
 #include <algorithm>
@@ -21,7 +21,7 @@ __global__ void k_fill_zero(uint32_t* p, int n)
     if (i < n) p[i] = 0u;
 }
 
-// particle_norm_maxima  simulation.py:142-159.  Values are >= 0 (or +inf), so
+// particle_norm_maxima  simulation.py:19-36.  Values are >= 0 (or +inf), so
 // their float bits order like unsigned integers.
 __global__ __launch_bounds__(kRedBlock) void k_norm_maxima(const float* __restrict__ v, int nv,
                                                            const float* __restrict__ a, int na, uint32_t* out)
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kRedBlock) void k_norm_maxima(const float* __restri
     }
 }
 
-// check_particle_finite  simulation.py:162-176
+// check_particle_finite  simulation.py:39-53
 __global__ __launch_bounds__(kRedBlock) void k_check_finite(const float* __restrict__ x, const float* __restrict__ v,
                                                             int n, const float* __restrict__ rho, int nd,
                                                             uint32_t* out)
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kRedBlock) void k_scene_maxima(const fs_scene* __re
     }
 }
 
-// DamBreakSimulation.step's dt policy per scene  (simulation.py:289-310)
+// DamBreakSimulation.step's dt policy per scene  (simulation.py:166-187)
 __global__ void k_scene_dt(fs_scene* scenes, int num_scenes, uint32_t* maxima, double stiffness, double exponent,
                            double rest_density, double viscosity, double frame_dt, int substeps, double* remaining,
                            int* status)

@@ -1,6 +1,6 @@
 """Device-resident dam-break frames: no host synchronisation per substep.
 
-The reference driver (``simulation.py:263-313``) reads the CFL maxima back to
+The reference driver (``simulation.py:140-190``) reads the CFL maxima back to
 the host before every WCSPH/DFSPH substep.  Here:
 
 * WCSPH keeps the plain solver but takes its dt from device memory.  The

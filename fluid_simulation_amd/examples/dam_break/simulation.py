@@ -1,10 +1,10 @@
 """Dam-break driver for the gfx950 solvers.
 
 Counterpart of the reference's ``examples/dam_break/simulation.py``
-(``DamBreakSimulation``, :179-364): the same model construction and material,
+(``DamBreakSimulation``, :56-241): the same model construction and material,
 the same per-method solver configs, the same CFL sub-stepping policy
-(:278-313), device reductions for the CFL maxima and the non-finite check
-(:142-176, :263-276, :350-358) and the same ``metrics``/``report`` dicts.
+(:155-190), device reductions for the CFL maxima and the non-finite check
+(:19-53, :140-153, :227-235) and the same ``metrics``/``report`` dicts.
 The APIC grid solver is not part of this path.
 """
 from __future__ import annotations
@@ -27,7 +27,7 @@ SOLVERS = {"wcsph": SolverWCSPH, "dfsph": SolverDFSPH, "pbf": SolverPBF}
 
 
 def solver_config(method: str, scene, *, wcsph_smoothing_length_coff=2.0, wcsph_clamp_negative_pressure=True):
-    """Per-method solver config of the dam break (simulation.py:233-255)."""
+    """Per-method solver config of the dam break (simulation.py:110-132)."""
     bounds = dict(bound_width=scene.spec.bound_hi[0], bound_height=scene.spec.bound_hi[1],
                   bound_length=scene.spec.bound_hi[2] * 0.5)
     common = dict(particle_radius=scene.radius, particle_length=scene.spacing, smoothing_length_coff=2.0,
@@ -45,7 +45,7 @@ def solver_config(method: str, scene, *, wcsph_smoothing_length_coff=2.0, wcsph_
 
 
 def material(method: str, scene, *, wcsph_stiffness=250000.0, wcsph_viscosity=1.0e-6):
-    """sph: material attributes of the dam break (simulation.py:203-215)."""
+    """sph: material attributes of the dam break (simulation.py:80-92)."""
     mat = {"rest_density": scene.spec.density, "surface_tension": 0.0, "viscosity": 1.0e-6}
     if method == "wcsph":
         mat.update(stiffness=wcsph_stiffness, exponent=7.0, viscosity=wcsph_viscosity)
@@ -99,7 +99,7 @@ class DamBreakSimulation:
         return ctypes.c_void_p(_lib.stream_handle(self.model.device))
 
     def _cfl_maxima(self):
-        """Reduce on the device; each CFL substep reads back only two scalars (simulation.py:263-276)."""
+        """Reduce on the device; each CFL substep reads back only two scalars (simulation.py:140-153)."""
         v = self.state_0.particle_qd
         use_accel = self.method == "wcsph" and self.total_substeps
         a = self.solver._particle_accel
@@ -116,7 +116,7 @@ class DamBreakSimulation:
         return math.sqrt(speed2), math.sqrt(accel2)
 
     def cfl_limit(self, speed: float, accel_max: float) -> float:
-        """Per-substep dt limit (simulation.py:290-304)."""
+        """Per-substep dt limit (simulation.py:167-181)."""
         limit = self.frame_dt / self.substeps
         if self.method == "wcsph":
             sound_speed = math.sqrt(self.material["stiffness"] * self.material["exponent"] / self.scene.spec.density)
@@ -131,7 +131,7 @@ class DamBreakSimulation:
         return limit
 
     def step(self):
-        """Advance a complete display frame; never discard unused CFL time (simulation.py:278-313)."""
+        """Advance a complete display frame; never discard unused CFL time (simulation.py:155-190)."""
         if self.method == "pbf":
             steps = self.substeps
             dt = self.frame_dt / steps
@@ -158,7 +158,7 @@ class DamBreakSimulation:
         self.total_substeps += 1
 
     def metrics(self):
-        """simulation.py:321-348"""
+        """simulation.py:198-225"""
         x, v = self.state_0.particle_q.numpy(), self.state_0.particle_qd.numpy()
         finite = bool(np.isfinite(x).all() and np.isfinite(v).all())
         if not finite:
@@ -185,7 +185,7 @@ class DamBreakSimulation:
         return result
 
     def check_finite(self):
-        """Validate on the device and read back a single integer (simulation.py:350-358)."""
+        """Validate on the device and read back a single integer (simulation.py:227-235)."""
         x, v, rho = self.state_0.particle_q, self.state_0.particle_qd, self.state_0.sph.rho
         _lib.check(_lib.load().fs_check_finite(ctypes.c_void_p(x.ptr), ctypes.c_void_p(v.ptr), x.size,
                                                ctypes.c_void_p(rho.ptr), rho.size,
@@ -195,7 +195,7 @@ class DamBreakSimulation:
             raise RuntimeError(f"Non-finite particle state at frame {self.frame}")
 
     def report(self):
-        """simulation.py:360-364"""
+        """simulation.py:237-241"""
         return dict(method=self.method, config=asdict(self.solver.config), material=self.material,
                     frame_dt=self.frame_dt, base_substeps=self.substeps,
                     time_step="fixed" if self.method == "pbf" else "CFL, complete frame",

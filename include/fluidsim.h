@@ -23,7 +23,7 @@
  *   - SolverDFSPH.step   solver/sph/solver_dfsph.py:598-898
  *   - SolverPBF.step     solver/pbf/solver_pbf.py:676-954
  *   - particle_norm_maxima / check_particle_finite
- *                        examples/dam_break/simulation.py:142-176
+ *                        examples/dam_break/simulation.py:19-53
  */
 #ifndef FLUIDSIM_H
 #define FLUIDSIM_H
@@ -388,7 +388,7 @@ typedef struct fs_batch_io {
 int32_t fs_batch_wcsph_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_scene* scenes,
                             int32_t num_scenes, const fs_batch_io* io, fs_stream_t stream);
 
-/* Per-scene CFL substep of the dam-break driver (simulation.py:289-310), on
+/* Per-scene CFL substep of the dam-break driver (simulation.py:166-187), on
  * the device, in float64 as the reference's host code evaluates it:
  *   limit = min(frame_dt/substeps, 0.25h/(c0+vmax), 0.25 sqrt(h/amax) if
  *           amax > 0, 0.025 h^2/viscosity if viscosity > 0),
@@ -407,7 +407,7 @@ int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_
                         int32_t* status, fs_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
-/* Driver diagnostics  (examples/dam_break/simulation.py:142-176)           */
+/* Driver diagnostics  (examples/dam_break/simulation.py:19-53)           */
 /* ------------------------------------------------------------------------ */
 
 /* maxima[0] = max |v|^2 over nv points, maxima[1] = max |a|^2 over na points

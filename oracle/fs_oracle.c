@@ -977,7 +977,7 @@ void ora_pbf_apply_bounds(int n, const int* flags, const int* mask, R* x, R* v, 
 /* Driver diagnostics  (examples/dam_break/simulation.py)                   */
 /* ------------------------------------------------------------------------ */
 
-/* particle_norm_maxima  simulation.py:142-159: max |v|^2, max |a|^2 with
+/* particle_norm_maxima  simulation.py:19-36: max |v|^2, max |a|^2 with
  * non-finite values mapped to +inf */
 void ora_norm_maxima(int nv, const R* v, int na, const R* a, R* maxima)
 {

@@ -11,7 +11,7 @@ launches, line by line:
 * ``dfsph_step``  -- ``solver/sph/solver_dfsph.py:598-898`` (host residuals
   ``:574-596`` evaluated with numpy exactly as the reference does)
 * ``pbf_step``    -- ``solver/pbf/solver_pbf.py:676-954``
-* ``cfl_dt`` / ``frame_substeps`` -- ``examples/dam_break/simulation.py:263-313``
+* ``cfl_dt`` / ``frame_substeps`` -- ``examples/dam_break/simulation.py:140-190``
 
 Every array is a numpy array in original particle order: positions and
 velocities ``(N, 3)``, scalars ``(N,)``.  ``precision`` is ``"f32"`` (the
@@ -589,7 +589,7 @@ def pbf_step(model: OModel, state_in: OState, cfg: PbfConfig, dt: float, precisi
 
 
 def norm_maxima(v: np.ndarray, a: np.ndarray | None, precision="f32"):
-    """particle_norm_maxima  simulation.py:142-159 -> (max|v|^2, max|a|^2)."""
+    """particle_norm_maxima  simulation.py:19-36 -> (max|v|^2, max|a|^2)."""
     lib = _lib(precision)
     v = np.ascontiguousarray(v, lib._np)
     a = np.zeros((0, 3), lib._np) if a is None else np.ascontiguousarray(a, lib._np)
@@ -600,7 +600,7 @@ def norm_maxima(v: np.ndarray, a: np.ndarray | None, precision="f32"):
 
 def cfl_limit(method, frame_dt, substeps, speed, accel_max, *, h, spacing, stiffness=None, exponent=None,
               rest_density=1000.0, viscosity=0.0):
-    """The per-substep dt limit  simulation.py:289-304."""
+    """The per-substep dt limit  simulation.py:166-181."""
     limit = frame_dt / substeps
     if method == "wcsph":
         sound_speed = math.sqrt(stiffness * exponent / rest_density)

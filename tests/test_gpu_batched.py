@@ -1,6 +1,6 @@
 """Batched WCSPH on the GPU: every scene of a batch evolves bit-identically to
 a plain solve of that scene with the same dt sequence, and the device CFL dt
-matches the dam-break driver's host formula (simulation.py:289-310)."""
+matches the dam-break driver's host formula (simulation.py:166-187)."""
 import numpy as np
 import pytest
 import torch
