@@ -17,7 +17,8 @@ oracle's FP32 restatement of the same WCSPH substep on the host cores.
 The default run also adds, under ``extras``: the 64-scene batched substep
 (graph-replayed, eager and 64 sequential solves), the differentiable WCSPH
 rollout at 50,540 particles x 200 substeps (forward, forward + backward), and
-whole WCSPH frames with the host CFL driver beside the device-resident one.
+whole WCSPH frames with the host CFL driver beside the device-resident one,
+and the APIC particle-separation pass over the jittered dam-break lattice.
 ``roofline.traffic`` is the HBM bytes per launch of the dominant kernel, read
 from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary for the launch
 variant the autotuner selected.
@@ -31,6 +32,7 @@ Other modes (each prints its own JSON lines and exits):
   --adjoint [--no-fd]        the differentiable rollout, with an FP64 central
                              finite-difference check unless --no-fd
   --frames N                 N whole WCSPH frames, host and device-resident drivers
+  --separation               the APIC particle-separation pass alone
   --copy-calibration         a 256 MiB device copy (the PMC byte-counter calibration)
 """
 from __future__ import annotations
@@ -103,6 +105,8 @@ def parse():
     ap.add_argument("--no-fd", action="store_true", help="skip the FD check of --adjoint")
     ap.add_argument("--frames", type=int, default=0, help="run only the whole-frame comparison (host CFL driver "
                                                           "vs device-resident) over this many frames")
+    ap.add_argument("--separation", action="store_true", help="run only the APIC particle separation "
+                                                                "(401,856 particles, 2 iterations)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants (no autotuning)")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and differentiable forms in the "
                                                              "default run")
@@ -544,6 +548,81 @@ def frames(particles=400000, count=3, chunk=16):
     return {"frames": out}
 
 
+# separation: per particle-iteration, grid 28 B + pass (perm 4, x 12, flags 4, x written 12) 32 B
+SEPARATION_BYTES = 60
+
+
+def separation(particles=400000, reps=10, cpu=True):
+    """APIC particle separation over the dam-break lattice jittered by
+    +-0.25 spacing (pairs closer than the rest distance everywhere), the APIC
+    dam-break config (tank bounds, clamp_eps 1e-3, separation defaults: rest =
+    2 x mean radius, 2 iterations).  One apply = 2 x (grid build + fused
+    accumulate/apply pass).  Timed on positions reset before each apply."""
+    import numpy as np
+    import torch
+
+    from fluid_simulation_amd import _lib
+    from fluid_simulation_amd.examples.dam_break.scene import make_scene
+    from fluid_simulation_amd.newton_compat import Axis, ModelBuilder
+    from fluid_simulation_amd.separation import ParticleSeparation, SeparationConfig
+
+    scene = make_scene(particles)
+    rng = np.random.default_rng(11)
+    pts = (scene.positions + rng.uniform(-0.25, 0.25, scene.positions.shape) * scene.spacing).astype(np.float32)
+    n = pts.shape[0]
+    b = ModelBuilder(up_axis=Axis.Z)
+    b.add_particles(pos=pts, vel=np.zeros_like(pts), mass=[scene.mass] * n, radius=[scene.radius] * n)
+    model = b.finalize(device="cuda:0")
+    cfg = SeparationConfig(bound_lo=scene.spec.bound_lo, bound_hi=scene.spec.bound_hi, clamp_eps=1.0e-3)
+    sep = ParticleSeparation(model, cfg)
+    state = model.state()
+    x0 = state.particle_q.tensor.clone()
+    for _ in range(2):
+        state.particle_q.tensor.copy_(x0)
+        sep.apply(state)
+    torch.cuda.synchronize()
+    lib = _lib.load()
+    lib.fs_profile_reset()
+    total = 0.0
+    for _ in range(reps):
+        state.particle_q.tensor.copy_(x0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lib.fs_profile_enable(1)
+        sep.apply(state)
+        lib.fs_profile_enable(0)
+        total += time.perf_counter() - t0
+    iters = sep.params().iterations
+    prof = {}
+    for name, (cnt, ms) in _lib.profile_report().items():
+        key = re.sub(r"<.*>", "", name.strip("()"))
+        c0, m0 = prof.get(key, (0, 0.0))
+        prof[key] = (c0 + cnt, m0 + ms)
+    build_ms = sum(prof[k][1] for k in ("k_hash_count", "k_bucket_scan", "k_scatter", "k_fix_gather")) / reps
+    pass_cnt, pass_ms = prof["k_separation"]
+    pass_us = pass_ms / pass_cnt * 1e3
+    gpu_ms = build_ms + pass_ms / reps
+    out = {"particles": n, "iterations": iters, "grid_dims": list(sep.dims), "rest_dist": sep.rest_dist,
+           "ms_per_apply": total / reps * 1e3, "kernel_ms_per_apply": gpu_ms,
+           "particle_iterations_per_s": n * iters / (gpu_ms / 1e3),
+           "separation_pass_us": pass_us, "grid_build_us_per_iteration": build_ms / iters * 1e3,
+           "pass_GBs_alg": 32 * n / (pass_us / 1e6) / 1e9,
+           "GBs_alg": SEPARATION_BYTES * n * iters / (gpu_ms / 1e3) / 1e9}
+    if cpu:
+        from oracle import oracle as O
+        O.set_threads(16)
+        p = O.separation_params(cfg, model.particle_radius.numpy())
+        flags = model.particle_flags.numpy()
+        t0 = time.perf_counter()
+        O.particle_separation(pts, flags, p, precision="f32")
+        t_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * iters / t_cpu, "unit": "particle-iterations/s", "cores": 16,
+                               "kind": "port", "sample": "one apply (2 iterations) over all particles"}
+    del sep, model, state
+    torch.cuda.empty_cache()
+    return {"separation": out}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -560,6 +639,9 @@ def main():
         return
     if args.batched:
         print(json.dumps(batched(args.batched, args.steps, args.warmup)), flush=True)
+        return
+    if args.separation:
+        print(json.dumps(separation(reps=max(args.steps, 5))), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path does not shard "
@@ -606,6 +688,8 @@ def main():
         extras.update(adjoint(fd=False, reps=2))
         print("# whole frames: host CFL driver vs device-resident", file=sys.stderr, flush=True)
         extras.update(frames(count=2))
+        print("# APIC particle separation: 401,856 particles, 2 iterations", file=sys.stderr, flush=True)
+        extras.update(separation(cpu=not args.no_cpu_baseline))
     cpu = None
     if not args.no_cpu_baseline:
         print("# cpu baseline (oracle)", file=sys.stderr, flush=True)

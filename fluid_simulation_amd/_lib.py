@@ -104,6 +104,13 @@ class FsPbfIO(ctypes.Structure):
     ]
 
 
+class FsSeparationParams(ctypes.Structure):
+    _fields_ = [
+        ("rest_dist", c_float), ("query_radius", c_float), ("strength", c_float), ("wall_band", c_float),
+        ("wall_margin", c_float), ("bound_lo", c_float * 3), ("bound_hi", c_float * 3), ("iterations", c_int32),
+    ]
+
+
 class FsScene(ctypes.Structure):
     _fields_ = [
         ("particle_offset", c_int32), ("particle_count", c_int32), ("bucket_base", c_int32), ("dims", c_int32 * 3),
@@ -151,6 +158,8 @@ SIGNATURES = {
                                 POINTER(FsDfsphIO), c_void_p]),
     "fs_pbf_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
                               POINTER(FsPbfIO), c_void_p]),
+    "fs_particle_separation": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsSeparationParams), c_void_p,
+                                         c_void_p, c_void_p, c_int32, c_void_p]),
     "fs_batch_wcsph_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), c_void_p, c_int32,
                                       POINTER(FsBatchIO), c_void_p]),
     "fs_batch_cfl_dt": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_double, c_double,

@@ -349,15 +349,17 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int* __r
     point_cells[f] = c;
     if (ga.pos) {
         v3 x = ld3(ga.x, i);
-        float m = ga.mass[i];
+        float m = ga.mass ? ga.mass[i] : 0.0f;
         ga.pos[f] = make_float4(x.x, x.y, x.z, m);
-        float rd = ga.rest_density[i];
-        // compute_particle_volume (solver_wcsph.py:109-116) times rest density,
-        // as compute_density forms mass_nei (solver_wcsph.py:148)
-        ga.wvol[f] = rd * (m / fmax_(rd, 1.0e-6f));
+        if (ga.wvol) {
+            float rd = ga.rest_density[i];
+            // compute_particle_volume (solver_wcsph.py:109-116) times rest density,
+            // as compute_density forms mass_nei (solver_wcsph.py:148)
+            ga.wvol[f] = rd * (m / fmax_(rd, 1.0e-6f));
+        }
         int fl = ga.flags[i];
         int act = fl & 1;
-        ga.sflags[f] = act | ((act && ga.mask[i] == 0) ? kDynamic : 0);
+        ga.sflags[f] = act | ((act && (!ga.mask || ga.mask[i] == 0)) ? kDynamic : 0);
         if (ga.any_inactive && __ballot(!act) && (threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1)
             atomicOr(ga.any_inactive, 1);
         if (ga.scene_of) ga.sscene[f] = ga.scene_of[i];

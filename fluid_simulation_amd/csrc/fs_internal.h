@@ -38,13 +38,13 @@ struct PreOp {
 struct GatherOp {
     const float* x = nullptr;      // positions in original order (what was bucketed)
     const float* v = nullptr;      // optional velocity source
-    const float* mass = nullptr;
-    const float* rest_density = nullptr;
+    const float* mass = nullptr;           // optional (pos.w = 0 without it)
+    const float* rest_density = nullptr;   // read only when wvol is set
     const int* flags = nullptr;
-    const int* mask = nullptr;
+    const int* mask = nullptr;             // optional (no mask: dynamic = active)
     float4* pos = nullptr;         // x,y,z,mass
     float4* vel = nullptr;         // vx,vy,vz,0   (when v != nullptr)
-    float* wvol = nullptr;         // rest_density * volume
+    float* wvol = nullptr;         // optional: rest_density * volume
     int* sflags = nullptr;         // bit0 active, bit1 dynamic
     const int* scene_of = nullptr; // optional: scene ids, gathered to sscene
     int* sscene = nullptr;

@@ -1173,6 +1173,65 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_xsph(PbfXsphArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// APIC particle separation
+// ---------------------------------------------------------------------------
+
+struct SepArgs {
+    PassCommon c;        // c.h: query radius (= cell width); c.cut2: rest_dist^2 with slack
+    float rest_dist;
+    float strength;
+    float wall_band;
+    float lo[3], hi[3];  // box corners
+    float clo[3], chi[3];  // clamp bounds: lo + wall_margin, hi - wall_margin
+    float* x;            // positions [n,3], original order, updated in place
+    float* delta;        // optional [n,3]: the displacement applied
+};
+
+// accumulate_particle_separation + apply_particle_separation_delta
+// (solver_apic.py:193-268), fused.  Positions are read only from the sorted
+// copy the grid build gathered, so particle i's new position is written in
+// place while other lanes still read its old one: the reference's two
+// launches (all displacements, then all updates) in one.
+template <int BLOCK, int CAP>
+__global__ __launch_bounds__(BLOCK) void k_separation(SepArgs a)
+{
+    __shared__ int lds[BLOCK * CAP];
+    PassCommon c = a.c;
+    const int s = pass_slot_b<BLOCK>();
+    if (s >= c.n) return;
+    const int i = c.perm[s];
+    v3 disp = V(0.0f, 0.0f, 0.0f);
+    if (c.sflags[s] & kActive) {
+        const v3 xi = xyz(c.pos[s]);
+        const float rest = a.rest_dist, strength = a.strength;
+        float count = 0.0f;
+        SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
+            if (k == s || !(c.sflags[k] & kActive)) return;
+            const v3 rij = vsub(xi, xyz(c.pos[k]));
+            const float d = vlen(rij);
+            if (d > 1.0e-8f && d < rest) {
+                disp = vadd(disp, muls(vdivs(rij, d), strength * (rest - d)));
+                count += 1.0f;
+            }
+        });
+        if (count > 0.0f) disp = vdivs(disp, count);
+        else disp = V(0.0f, 0.0f, 0.0f);
+        // into-wall components removed near the walls (:233-245)
+        if (xi.x <= a.lo[0] + a.wall_band && disp.x < 0.0f) disp.x = 0.0f;
+        if (xi.x >= a.hi[0] - a.wall_band && disp.x > 0.0f) disp.x = 0.0f;
+        if (xi.y <= a.lo[1] + a.wall_band && disp.y < 0.0f) disp.y = 0.0f;
+        if (xi.y >= a.hi[1] - a.wall_band && disp.y > 0.0f) disp.y = 0.0f;
+        if (xi.z <= a.lo[2] + a.wall_band && disp.z < 0.0f) disp.z = 0.0f;
+        if (xi.z >= a.hi[2] - a.wall_band && disp.z > 0.0f) disp.z = 0.0f;
+        // wp.clamp(v, lo, hi) = min(max(lo, v), hi)  (:262-267)
+        const v3 p = vadd(xi, disp);
+        st3(a.x, i, V(fmin_(fmax_(a.clo[0], p.x), a.chi[0]), fmin_(fmax_(a.clo[1], p.y), a.chi[1]),
+                      fmin_(fmax_(a.clo[2], p.z), a.chi[2])));
+    }
+    if (a.delta) st3(a.delta, i, disp);
+}
+
+// ---------------------------------------------------------------------------
 // launch variants of the tunable WCSPH passes: workgroup size and the depth of
 // the per-lane LDS survivor list (which bounds occupancy through LDS).  Every
 // variant visits the same candidates in the same order, so results are
@@ -1804,6 +1863,57 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     fa.control = w->control;
     fa.iterations = io->iterations;
     FS_LAUNCH(k_dfsph_finalize, dim3(blocks_for(n, kElemBlock)), dim3(kElemBlock), 0, s, fa);
+    return FS_OK;
+}
+
+int32_t fs_particle_separation(fs_grid* g, fs_work* w, const fs_separation_params* prm, float* positions,
+                               const int32_t* flags, float* delta, int32_t n, fs_stream_t stream)
+{
+    if (!g || !w || !prm) {
+        set_error("fs_particle_separation: null grid, workspace or params");
+        return FS_ERR_ARG;
+    }
+    if (n < 0 || n > g->capacity || n > w->capacity) {
+        set_error("particle count %d exceeds grid capacity %d / workspace capacity %d", n, g->capacity, w->capacity);
+        return FS_ERR_CAPACITY;
+    }
+    if (n == 0) return FS_OK;
+    if (!positions || !flags) {
+        set_error("fs_particle_separation: missing positions or flags");
+        return FS_ERR_ARG;
+    }
+    if (!(prm->query_radius > 0.0f) || prm->iterations < 1) {
+        set_error("fs_particle_separation: query_radius must be positive and iterations >= 1");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    GatherOp ga;
+    ga.x = positions;
+    ga.flags = flags;
+    ga.pos = reinterpret_cast<float4*>(w->pos);
+    ga.sflags = w->flags;
+    SepArgs a;
+    a.rest_dist = prm->rest_dist;
+    a.strength = prm->strength;
+    a.wall_band = prm->wall_band;
+    for (int d = 0; d < 3; ++d) {
+        a.lo[d] = prm->bound_lo[d];
+        a.hi[d] = prm->bound_hi[d];
+        a.clo[d] = prm->bound_lo[d] + prm->wall_margin;
+        a.chi[d] = prm->bound_hi[d] - prm->wall_margin;
+    }
+    a.x = positions;
+    a.delta = delta;
+    for (int it = 0; it < prm->iterations; ++it) {
+        PreOp pre;
+        pre.x_in = positions;
+        int32_t rc = build_grid(g, n, prm->query_radius, pre, ga, s);   // :1036
+        if (rc) return rc;
+        a.c = make_common(g, w, reinterpret_cast<const float4*>(w->pos), prm->query_radius, kListNone);
+        a.c.cut2 = prm->rest_dist * prm->rest_dist * kCutSlack;
+        FS_LAUNCH((k_separation<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s,
+                  a);   // :1037-1066
+    }
     return FS_OK;
 }
 

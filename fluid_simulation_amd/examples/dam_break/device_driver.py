@@ -27,7 +27,10 @@ from .simulation import DamBreakSimulation
 
 
 class DeviceResidentDamBreak:
-    def __init__(self, scene, method: str = "wcsph", device="cuda:0", substeps: int = 4, chunk: int = 16):
+    def __init__(self, scene, method: str = "wcsph", device="cuda:0", substeps: int = 4, chunk: int = 16,
+                 **sim_kwargs):
+        """``sim_kwargs`` go to ``DamBreakSimulation`` (the WCSPH material and
+        pressure-clamp options)."""
         if method not in ("wcsph", "pbf"):
             raise ValueError("device-resident frames cover wcsph and pbf; dfsph uses DamBreakSimulation")
         if chunk < 2 or chunk % 2:
@@ -40,7 +43,7 @@ class DeviceResidentDamBreak:
         self.chunk = int(chunk)
         self.total_substeps = 0
         self._graph = None
-        self.sim = DamBreakSimulation(scene, method, device=device, substeps=substeps)
+        self.sim = DamBreakSimulation(scene, method, device=device, substeps=substeps, **sim_kwargs)
         self.frame_dt = self.sim.frame_dt
         if method == "wcsph":
             self._cfl_setup()
@@ -124,8 +127,7 @@ class DeviceResidentDamBreak:
                     sim.total_substeps = total
                 self._graph.replay()
                 sim.total_substeps += self.substeps
-            self.frame += 1
-            self.total_substeps += self.substeps
+            self._end_frame(self.substeps)
             return self.substeps
         self.remaining.fill_(self.frame_dt)
         taken = 0
@@ -145,8 +147,13 @@ class DeviceResidentDamBreak:
                 raise RuntimeError("Non-finite velocity or WCSPH acceleration")
             if code == 2 or taken > 4096:
                 raise RuntimeError("CFL step cannot safely complete this frame")
+        self._end_frame(taken)
+        return taken
+
+    def _end_frame(self, taken: int):
+        sim = self.sim
         self.frame += 1
+        self.total_substeps += taken
         sim.frame = self.frame
         sim.sim_time = self.frame * self.frame_dt
-        self.total_substeps += taken
-        return taken
+        sim.last_substeps = taken

@@ -342,6 +342,33 @@ int32_t fs_pbf_step(fs_grid* grid, fs_work* work, const fs_model* model, const f
                     const fs_pbf_io* io, fs_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* APIC particle separation                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Host-side values of SolverAPIC._apply_particle_separation
+ * (solver/apic/solver_apic.py:1003-1066) and SolverAPIC.__init__ (:640-650). */
+typedef struct fs_separation_params {
+    float rest_dist;     /* push apart pairs closer than this                    */
+    float query_radius;  /* max(rest_dist, 1e-5): grid cell width, query radius  */
+    float strength;      /* separation_strength                                  */
+    float wall_band;     /* max(wall_push_band, 2 wall_margin): into-wall kill   */
+    float wall_margin;   /* positions clamped to [lo + margin, hi - margin]      */
+    float bound_lo[3];
+    float bound_hi[3];
+    int32_t iterations;  /* max(1, separation_iterations)                        */
+} fs_separation_params;
+
+/* `iterations` x (grid build at the current positions with cell width
+ * query_radius, accumulate_particle_separation, apply_particle_separation_delta;
+ * solver_apic.py:193-268, 1036-1066).  The grid is the caller's HashGrid of
+ * the box dims; the workspace may have list_capacity 0.  positions [n*3] are
+ * updated in place; delta [n*3] (optional) receives the last iteration's
+ * displacement (zero for inactive particles, which are not moved). */
+int32_t fs_particle_separation(fs_grid* grid, fs_work* work, const fs_separation_params* params,
+                               float* positions, const int32_t* flags, float* delta, int32_t n,
+                               fs_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Batched WCSPH: many independent scenes in one launch per pass            */
 /* ------------------------------------------------------------------------ */
 

@@ -998,6 +998,65 @@ void ora_norm_maxima(int nv, const R* v, int na, const R* a, R* maxima)
     maxima[1] = m1;
 }
 
+/* ------------------------------------------------------------------------ */
+/* APIC particle separation                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* accumulate_particle_separation  solver_apic.py:193-248.  One thread per
+ * original index i; candidates of hash_grid_query(grid, x_i, query_radius)
+ * in visit order; lo/hi are the box corners (float32 vec3 in the reference),
+ * wall_band the into-wall kill band. */
+void ora_separation_accumulate(const ora_grid* g, const int* flags, const R* x, R rest_dist, R query_radius,
+                               R strength, const R* lo, const R* hi, R wall_band, R* delta)
+{
+    PARFOR
+    for (int i = 0; i < g->num_points; ++i) {
+        if (!is_active(flags[i])) {
+            vstore(delta, i, VZERO);
+            continue;
+        }
+        v3 xi = vload(x, i);
+        v3 disp = VZERO;
+        R count = RC(0.0);
+        FOR_CANDIDATES(g, xi, query_radius, j, {
+            if (j != i && is_active(flags[j])) {
+                v3 rij = vsub(xi, vload(x, j));
+                R d = vlen(rij);
+                if (d > RC(1.0e-8) && d < rest_dist) {
+                    v3 nrm = vdivs(rij, d);
+                    disp = vadd(disp, muls(nrm, strength * (rest_dist - d)));
+                    count += RC(1.0);
+                }
+            }
+        });
+        if (count > RC(0.0)) disp = vdivs(disp, count);
+        else disp = VZERO;
+        if (xi.x <= lo[0] + wall_band && disp.x < RC(0.0)) disp.x = RC(0.0);
+        if (xi.x >= hi[0] - wall_band && disp.x > RC(0.0)) disp.x = RC(0.0);
+        if (xi.y <= lo[1] + wall_band && disp.y < RC(0.0)) disp.y = RC(0.0);
+        if (xi.y >= hi[1] - wall_band && disp.y > RC(0.0)) disp.y = RC(0.0);
+        if (xi.z <= lo[2] + wall_band && disp.z < RC(0.0)) disp.z = RC(0.0);
+        if (xi.z >= hi[2] - wall_band && disp.z > RC(0.0)) disp.z = RC(0.0);
+        vstore(delta, i, disp);
+    }
+}
+
+/* wp.clamp(x, a, b) = min(max(a, x), b) */
+static inline R clamp_(R v, R a, R b) { return rmin(rmax(a, v), b); }
+
+/* apply_particle_separation_delta  solver_apic.py:251-268 (in place). */
+void ora_separation_apply(int n, const int* flags, R* x, const R* delta, const R* lo, const R* hi, R wall_margin)
+{
+    const v3 l = V(lo[0] + wall_margin, lo[1] + wall_margin, lo[2] + wall_margin);
+    const v3 h = V(hi[0] - wall_margin, hi[1] - wall_margin, hi[2] - wall_margin);
+    PARFOR
+    for (int i = 0; i < n; ++i) {
+        if (!is_active(flags[i])) continue;
+        v3 p = vadd(vload(x, i), vload(delta, i));
+        vstore(x, i, V(clamp_(p.x, l.x, h.x), clamp_(p.y, l.y, h.y), clamp_(p.z, l.z, h.z)));
+    }
+}
+
 void ora_set_threads(int nthreads)
 {
 #ifdef _OPENMP

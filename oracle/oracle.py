@@ -12,6 +12,8 @@ launches, line by line:
   ``:574-596`` evaluated with numpy exactly as the reference does)
 * ``pbf_step``    -- ``solver/pbf/solver_pbf.py:676-954``
 * ``cfl_dt`` / ``frame_substeps`` -- ``examples/dam_break/simulation.py:140-190``
+* ``particle_separation`` -- ``solver/apic/solver_apic.py:193-268`` composed as
+  ``SolverAPIC._apply_particle_separation`` (``:1003-1066``)
 
 Every array is a numpy array in original particle order: positions and
 velocities ``(N, 3)``, scalars ``(N,)``.  ``precision`` is ``"f32"`` (the
@@ -612,3 +614,75 @@ def cfl_limit(method, frame_dt, substeps, speed, accel_max, *, h, spacing, stiff
     elif speed > 0.0:
         limit = min(limit, 0.4 * spacing / speed)
     return limit
+
+
+# --------------------------------------------------------------------------
+# APIC particle separation
+# --------------------------------------------------------------------------
+
+
+@dataclass
+class SeparationConfig:
+    """The ``SolverAPIC.Config`` fields the separation pass reads, with the
+    reference's defaults (solver/apic/solver_apic.py:557-582)."""
+    bound_lo: tuple = (-1.0, -1.0, 0.0)
+    bound_hi: tuple = (1.0, 1.0, 2.0)
+    clamp_eps: float = 1.0e-4
+    wall_margin: float = -1.0
+    wall_push_band: float = -1.0
+    separation_rest_dist: float = -1.0
+    separation_strength: float = 0.55
+    separation_iterations: int = 2
+
+
+def separation_params(cfg, radius) -> dict:
+    """Host-side values of the pass: wall margin and push band
+    (SolverAPIC.__init__, solver_apic.py:640-650), rest distance, query
+    radius, iterations and HashGrid dims (_apply_particle_separation
+    :1009-1027).  ``radius`` is the model's float32 particle radii."""
+    radius = np.asarray(radius, np.float32)
+    n = radius.shape[0]
+    if n > 0 and float(cfg.wall_margin) < 0.0:
+        wall_margin = float(np.mean(radius))
+    elif float(cfg.wall_margin) < 0.0:
+        wall_margin = float(cfg.clamp_eps)
+    else:
+        wall_margin = float(cfg.wall_margin)
+    wall_push_band = 1.5 * wall_margin if float(cfg.wall_push_band) < 0.0 else float(cfg.wall_push_band)
+    if float(cfg.separation_rest_dist) > 0.0:
+        rest = float(cfg.separation_rest_dist)
+    else:
+        # with no particles the reference returns before this (:1006-1007)
+        rest = float(2.0 * float(np.mean(radius))) if n > 0 else 0.0
+    query = max(rest, 1.0e-5)
+    lo = np.asarray(cfg.bound_lo, dtype=np.float64)
+    hi = np.asarray(cfg.bound_hi, dtype=np.float64)
+    extent = np.maximum(hi - lo, query)
+    dims = np.maximum(4, np.ceil(extent / query).astype(int))
+    return dict(rest_dist=rest, query_radius=query, strength=float(cfg.separation_strength),
+                iterations=max(1, int(cfg.separation_iterations)),
+                wall_band=max(float(wall_push_band), float(wall_margin) * 2.0), wall_margin=wall_margin,
+                dims=(int(dims[0]), int(dims[1]), int(dims[2])), bound_lo=tuple(cfg.bound_lo),
+                bound_hi=tuple(cfg.bound_hi))
+
+
+def particle_separation(x, flags, params: dict, precision="f32", round_mode=0):
+    """``iterations`` x (grid build at the current positions, accumulate,
+    apply).  Returns (positions after, delta of the last iteration)."""
+    lib = _lib(precision)
+    real, npt = lib._real, lib._np
+    x = np.array(x, dtype=npt, copy=True)
+    flags = np.ascontiguousarray(flags, np.int32)
+    lo = np.asarray(params["bound_lo"], npt)
+    hi = np.asarray(params["bound_hi"], npt)
+    delta = np.zeros_like(x)
+    grid = OracleGrid(*params["dims"], precision=precision, round_mode=round_mode)
+    for _ in range(params["iterations"]):
+        grid.build(x, params["query_radius"])
+        g = grid.struct()
+        lib.ora_separation_accumulate(ctypes.byref(g), _p(flags), _p(x), real(params["rest_dist"]),
+                                      real(params["query_radius"]), real(params["strength"]), _p(lo), _p(hi),
+                                      real(params["wall_band"]), _p(delta))
+        lib.ora_separation_apply(ctypes.c_int(x.shape[0]), _p(flags), _p(x), _p(delta), _p(lo), _p(hi),
+                                 real(params["wall_margin"]))
+    return x, delta

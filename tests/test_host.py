@@ -193,7 +193,7 @@ def test_struct_layouts_agree_with_header(tmp_path):
                "fs_dfsph_params": _lib.FsDfsphParams, "fs_dfsph_io": _lib.FsDfsphIO,
                "fs_pbf_params": _lib.FsPbfParams, "fs_pbf_io": _lib.FsPbfIO,
                "fs_scene": _lib.FsScene, "fs_batch_io": _lib.FsBatchIO,
-               "fs_wcsph_adjoint_io": _lib.FsWcsphAdjointIO}
+               "fs_wcsph_adjoint_io": _lib.FsWcsphAdjointIO, "fs_separation_params": _lib.FsSeparationParams}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fluidsim.h"', "int main(void){"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
