@@ -341,6 +341,48 @@ __device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, int base_bytes
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, base_bytes, imm, 0));
 }
 
+// Squared distance for the survivor prefilter only: fused multiply-adds (3
+// instructions instead of 5).  It differs from the reference's written-order
+// value by a few ulps, far inside the relative slack of the cuts it is
+// compared with (1e-4), so it never drops a candidate the exact test inside
+// the pass bodies would keep; the bodies still decide with the exact value.
+__device__ __forceinline__ float prefilter_d2(float dx, float dy, float dz)
+{
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// A two-phase neighbour body: load(k) fetches what the body needs of
+// neighbour k (unconditionally, no control flow), body(k, data) uses it.  The
+// loops fetch the data of several neighbours before running their bodies, so
+// those loads are in flight together instead of one round trip per
+// neighbour; the bodies still run one after another in visit order, so every
+// sum is formed in the same order as with a plain body(k).
+template <class Load, class Body>
+struct Phased {
+    Load load;
+    Body body;
+};
+template <class Load, class Body>
+__device__ __forceinline__ Phased<Load, Body> phased(Load load, Body body)
+{
+    return Phased<Load, Body>{load, body};
+}
+template <class T>
+struct IsPhased {
+    static constexpr bool value = false;
+};
+template <class Load, class Body>
+struct IsPhased<Phased<Load, Body>> {
+    static constexpr bool value = true;
+};
+template <class Body>
+__device__ __forceinline__ void run_body(Body& body, int k)
+{
+    if constexpr (IsPhased<typename std::decay<Body>::type>::value) body.body(k, body.load(k));
+    else body(k);
+}
+constexpr int kPhasedUnroll = 4;   // neighbours whose data is fetched together
+
 // Visit the candidates of hash_grid_query(p, r) in Warp order (returning how
 // many there were), but call
 // `body(k)` (k = sorted slot) only for those with squared distance not above
@@ -394,7 +436,7 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
                         if (u < take) {
                             count(k0 + u);
                             float ddx = px - q[u].x, ddy = py - q[u].y, ddz = pz - q[u].z;
-                            float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            float d2 = prefilter_d2(ddx, ddy, ddz);
                             if (!(d2 > keep2)) list[(cnt++) * BLOCK] = k0 + u;
                         }
                     }
@@ -407,7 +449,7 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
                     for (int u = 0; u < kBatch; ++u) {
                         if (kCounts && u < take) count(k0 + u);
                         float ddx = px - q[u].x, ddy = py - q[u].y, ddz = pz - q[u].z;
-                        float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        float d2 = prefilter_d2(ddx, ddy, ddz);
                         list[cnt * BLOCK] = k0 + u;
                         cnt += (u < take && !(d2 > keep2)) ? 1 : 0;
                     }
@@ -415,16 +457,34 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
                 it.k = k0 + take;
                 candidates += take;
             }
-            for (int e = 0; e < cnt; ++e) {
-                const int k = list[e * BLOCK];
-                if (record) {
-                    if (written < L.cap) {
-                        *out = k;
-                        out += L.n;
-                    }
-                    ++written;
+            auto rec = [&](int k) {
+                if (written < L.cap) {
+                    *out = k;
+                    out += L.n;
                 }
-                body(k);
+                ++written;
+            };
+            int e = 0;
+            if constexpr (IsPhased<typename std::decay<Body>::type>::value) {
+                for (; e + kPhasedUnroll <= cnt; e += kPhasedUnroll) {
+                    int k[kPhasedUnroll];
+#pragma unroll
+                    for (int u = 0; u < kPhasedUnroll; ++u) k[u] = list[(e + u) * BLOCK];
+                    if (record) {
+#pragma unroll
+                        for (int u = 0; u < kPhasedUnroll; ++u) rec(k[u]);
+                    }
+                    decltype(body.load(0)) d[kPhasedUnroll];
+#pragma unroll
+                    for (int u = 0; u < kPhasedUnroll; ++u) d[u] = body.load(k[u]);
+#pragma unroll
+                    for (int u = 0; u < kPhasedUnroll; ++u) body.body(k[u], d[u]);
+                }
+            }
+            for (; e < cnt; ++e) {
+                const int k = list[e * BLOCK];
+                if (record) rec(k);
+                run_body(body, k);
             }
             if (done) break;
         }
@@ -457,12 +517,23 @@ __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4
                     const int* __restrict__ idx2 = idx + 2 * (ptrdiff_t)n;
                     const int k2 = idx2[0], k3 = idx2[n];
                     idx = idx2 + 2 * (ptrdiff_t)n;
-                    body(k0);
-                    body(k1);
-                    body(k2);
-                    body(k3);
+                    if constexpr (IsPhased<typename std::decay<Body>::type>::value) {
+                        const auto d0 = body.load(k0);
+                        const auto d1 = body.load(k1);
+                        const auto d2 = body.load(k2);
+                        const auto d3 = body.load(k3);
+                        body.body(k0, d0);
+                        body.body(k1, d1);
+                        body.body(k2, d2);
+                        body.body(k3, d3);
+                    } else {
+                        body(k0);
+                        body(k1);
+                        body(k2);
+                        body(k3);
+                    }
                 }
-                for (; e < m; ++e, idx += n) body(*idx);
+                for (; e < m; ++e, idx += n) run_body(body, *idx);
                 return 0;
             }
         }

@@ -179,33 +179,39 @@ __device__ __forceinline__ int hash_point(const HashArgs& a, int i)
 }
 
 // Bucket every point and take a slot rank from its bucket counter.  Lanes are
-// fed in the previous build's sorted order when there is one, so a wave holds
-// few distinct buckets; the counter increments are aggregated per bucket and
-// wave (one atomic per distinct bucket).  Ranks only need to be distinct:
-// k_fix_gather orders each bucket by original index afterwards.
+// fed in the previous build's sorted order when there is one, so equal buckets
+// sit in runs of adjacent lanes: the first lane of each run adds the run's
+// length to the counter (one atomic instruction for the wave, whatever the
+// number of distinct buckets) and the run's lanes take consecutive ranks from
+// it.  A bucket split over several runs just takes several atomics.  Ranks only
+// need to be distinct: k_fix_gather orders each bucket by original index
+// afterwards.
 template <int MODE>
 __global__ __launch_bounds__(kElemBlock) void k_hash_count(HashArgs a)
 {
     const int t = blockIdx.x * kElemBlock + threadIdx.x;
     if (a.pre.control_reset && t < 16) a.pre.control_reset[t] = 0;
-    const bool valid = t < a.n;
+    const bool valid = t < a.n;   // the valid lanes of a wave are a prefix
     const int i = valid ? (a.order ? a.order[t] : t) : 0;
     int c = -1;
     if (valid) c = hash_point<MODE>(a, i);
     const int lane = threadIdx.x & 63;
-    unsigned long long pending = __ballot(valid);
-    int rank = 0;
-    while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int b = __shfl(c, leader, 64);
-        const unsigned long long same = __ballot(valid && c == b) & pending;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&a.count[b], __popcll(same));
-        base = __shfl(base, leader, 64);
-        if ((same >> lane) & 1ull) rank = base + __popcll(same & ((1ull << lane) - 1ull));
-        pending &= ~same;
+    const int prev = __shfl_up(c, 1, 64);
+    const bool head = valid && (lane == 0 || prev != c);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long valids = __ballot(valid);
+    const unsigned long long below = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);   // lanes 0..lane
+    int base = 0;
+    if (head) {
+        const unsigned long long later = heads & ~below;
+        const int next = later ? __ffsll((long long)later) - 1 : 64;
+        const unsigned long long run = (next == 64 ? ~0ull : ((1ull << next) - 1ull)) & ~((1ull << lane) - 1ull);
+        base = atomicAdd(&a.count[c], __popcll(run & valids));
     }
-    if (valid) a.rank[i] = rank;
+    const unsigned long long mine = heads & below;
+    const int lead = mine ? 63 - __clzll((long long)mine) : 0;
+    const int lead_base = __shfl(base, lead, 64);
+    if (valid) a.rank[i] = lead_base + (lane - lead);
 }
 
 

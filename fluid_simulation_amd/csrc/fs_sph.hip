@@ -168,6 +168,13 @@ struct WcsphDensityArgs {
 
 // compute_density_pressure  solver_wcsph.py:119-158, for lane s with the
 // pass parameters in `c`
+// what the density body reads of a neighbour
+struct DensityNbr {
+    int flags;
+    float4 x;
+    float wvol;
+};
+
 template <int BLOCK, int CAP>
 __device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, const PassCommon& c, int clamp, int s,
                                                    int* lds)
@@ -184,12 +191,13 @@ __device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, co
     }
     const float h = c.h;
     float rho_temp = 0.0f;
-    SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-        if (!(c.sflags[k] & kActive)) return;
-        v3 dist = vsub(xyz(xi), xyz(c.pos[k]));
-        float mass_nei = a.wvol[k];
-        rho_temp += mass_nei * cubic_w(vlen(dist), h);
-    });
+    SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+        [&](int k) { return DensityNbr{c.sflags[k], c.pos[k], a.wvol[k]}; },
+        [&](int k, const DensityNbr& nb) {
+            if (!(nb.flags & kActive)) return;
+            v3 dist = vsub(xyz(xi), xyz(nb.x));
+            rho_temp += nb.wvol * cubic_w(vlen(dist), h);
+        }));
     const float rho_i = rho_temp;
     float p = a.stiffness[i] * (fpow_(rho_i / fmax_(a.rest_density[i], 1.0e-6f), a.exponent[i]) - 1.0f);
     if (clamp) p = fmax_(p, 0.0f);
@@ -268,6 +276,15 @@ struct WcsphForceArgs {
 // p/max(rho^2,1e-6), m/max(rho,1e-6) precomputed -- the same float operations
 // on the same operands.  A zero cohesion (gamma) or viscosity (mu) coefficient
 // contributes an exact zero, so its term is skipped.
+// what the force body reads of a neighbour: flags, x + mass, v + m/rho,
+// p/max(rho^2)
+struct ForceNbr {
+    int flags;
+    float4 x;
+    float4 v;
+    float dp;
+};
+
 template <int BLOCK, int CAP>
 __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const PassCommon& c, float pr, float dt,
                                                  float dmp, float sx, float sy, float sz, int s, int* lds)
@@ -289,24 +306,25 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
     const float visc_den0 = 0.01f * (h * h);
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
-    REPLAY_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-        if (k == s || !(c.sflags[k] & kActive)) return;
-        const float4 xj = c.pos[k];
-        const float4 sj = a.sca[k];
-        const v3 d = vsub(xi, xyz(xj));
-        const float distance = vlen(d);
-        const float mass_nei = xj.w;
-        const v3 grad = cubic_grad_r(d, distance, h);
-        if (gamma != 0.0f)
-            acc = vsub(acc, smul(gamma / m_i * mass_nei, muls(d, cubic_w(fmax_(distance, pr), h))));
-        if (mu != 0.0f) {
-            const float4 vj = a.vel[k];
-            float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
-            float sv = mu10 * vj.w * v_current_nei / (distance * distance + visc_den0);
-            acc = vadd(acc, smul(sv, grad));
-        }
-        acc = vadd(acc, smul(-mass_nei * (dp_i + sj.z), grad));
-    });
+    REPLAY_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+        [&](int k) { return ForceNbr{c.sflags[k], c.pos[k], a.vel[k], a.sca[k].z}; },
+        [&](int k, const ForceNbr& nb) {
+            if (k == s || !(nb.flags & kActive)) return;
+            const float4 xj = nb.x;
+            const v3 d = vsub(xi, xyz(xj));
+            const float distance = vlen(d);
+            const float mass_nei = xj.w;
+            const v3 grad = cubic_grad_r(d, distance, h);
+            if (gamma != 0.0f)
+                acc = vsub(acc, smul(gamma / m_i * mass_nei, muls(d, cubic_w(fmax_(distance, pr), h))));
+            if (mu != 0.0f) {
+                const float4 vj = nb.v;
+                float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
+                float sv = mu10 * vj.w * v_current_nei / (distance * distance + visc_den0);
+                acc = vadd(acc, smul(sv, grad));
+            }
+            acc = vadd(acc, smul(-mass_nei * (dp_i + nb.dp), grad));
+        }));
     int w = a.world[i];
     w = w > 0 ? w : 0;
     const v3 acc_g = vadd(acc, ld3(a.gravity, w));
