@@ -51,6 +51,29 @@ struct PassCommon {
     NbrList L;
 };
 
+// What a two-phase neighbour body fetches of neighbour k (sorted slot):
+// its flags and position (+ mass in w), plus the pass's own neighbour data.
+struct NbrX {
+    int flags;
+    float4 x;
+};
+struct NbrXS {   // + one scalar
+    int flags;
+    float4 x;
+    float s;
+};
+struct NbrXV {   // + one float4
+    int flags;
+    float4 x;
+    float4 v;
+};
+struct NbrXVS {  // + a float4 and a scalar
+    int flags;
+    float4 x;
+    float4 v;
+    float s;
+};
+
 __device__ __forceinline__ int pass_slot() { return xcd_tile() * kPassBlock + threadIdx.x; }
 template <int BLOCK>
 __device__ __forceinline__ int pass_slot_b() { return xcd_tile() * BLOCK + threadIdx.x; }
@@ -168,13 +191,6 @@ struct WcsphDensityArgs {
 
 // compute_density_pressure  solver_wcsph.py:119-158, for lane s with the
 // pass parameters in `c`
-// what the density body reads of a neighbour
-struct DensityNbr {
-    int flags;
-    float4 x;
-    float wvol;
-};
-
 template <int BLOCK, int CAP>
 __device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, const PassCommon& c, int clamp, int s,
                                                    int* lds)
@@ -192,11 +208,11 @@ __device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, co
     const float h = c.h;
     float rho_temp = 0.0f;
     SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
-        [&](int k) { return DensityNbr{c.sflags[k], c.pos[k], a.wvol[k]}; },
-        [&](int k, const DensityNbr& nb) {
+        [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.wvol[k]}; },
+        [&](int k, const NbrXS& nb) {
             if (!(nb.flags & kActive)) return;
             v3 dist = vsub(xyz(xi), xyz(nb.x));
-            rho_temp += nb.wvol * cubic_w(vlen(dist), h);
+            rho_temp += nb.s * cubic_w(vlen(dist), h);
         }));
     const float rho_i = rho_temp;
     float p = a.stiffness[i] * (fpow_(rho_i / fmax_(a.rest_density[i], 1.0e-6f), a.exponent[i]) - 1.0f);
@@ -276,15 +292,6 @@ struct WcsphForceArgs {
 // p/max(rho^2,1e-6), m/max(rho,1e-6) precomputed -- the same float operations
 // on the same operands.  A zero cohesion (gamma) or viscosity (mu) coefficient
 // contributes an exact zero, so its term is skipped.
-// what the force body reads of a neighbour: flags, x + mass, v + m/rho,
-// p/max(rho^2)
-struct ForceNbr {
-    int flags;
-    float4 x;
-    float4 v;
-    float dp;
-};
-
 template <int BLOCK, int CAP>
 __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const PassCommon& c, float pr, float dt,
                                                  float dmp, float sx, float sy, float sz, int s, int* lds)
@@ -307,8 +314,8 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
     REPLAY_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
-        [&](int k) { return ForceNbr{c.sflags[k], c.pos[k], a.vel[k], a.sca[k].z}; },
-        [&](int k, const ForceNbr& nb) {
+        [&](int k) { return NbrXVS{c.sflags[k], c.pos[k], a.vel[k], a.sca[k].z}; },
+        [&](int k, const NbrXVS& nb) {
             if (k == s || !(nb.flags & kActive)) return;
             const float4 xj = nb.x;
             const v3 d = vsub(xi, xyz(xj));
@@ -323,7 +330,7 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
                 float sv = mu10 * vj.w * v_current_nei / (distance * distance + visc_den0);
                 acc = vadd(acc, smul(sv, grad));
             }
-            acc = vadd(acc, smul(-mass_nei * (dp_i + nb.dp), grad));
+            acc = vadd(acc, smul(-mass_nei * (dp_i + nb.s), grad));
         }));
     int w = a.world[i];
     w = w > 0 ? w : 0;
@@ -617,16 +624,17 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
     float rho_temp = 0.0f;
     v3 sum_grad = V(0.0f, 0.0f, 0.0f);
     float sum_grad_sq = 0.0f;
-    auto body = [&](int k) {
-        if (!(a.c.sflags[k] & kActive)) return;
-        const float4 xj = a.c.pos[k];
-        v3 r = vsub(xi, xyz(xj));
-        const float rn = vlen(r);
-        rho_temp += a.wvol[k] * cubic_w(rn, h);
-        v3 grad = smul(xj.w, cubic_grad_r(r, rn, h));
-        sum_grad = vadd(sum_grad, grad);
-        sum_grad_sq += vdot(grad, grad);
-    };
+    auto body = phased([&](int k) { return NbrXS{a.c.sflags[k], a.c.pos[k], a.wvol[k]}; },
+                       [&](int k, const NbrXS& nb) {
+                           if (!(nb.flags & kActive)) return;
+                           const float4 xj = nb.x;
+                           v3 r = vsub(xi, xyz(xj));
+                           const float rn = vlen(r);
+                           rho_temp += nb.s * cubic_w(rn, h);
+                           v3 grad = smul(xj.w, cubic_grad_r(r, rn, h));
+                           sum_grad = vadd(sum_grad, grad);
+                           sum_grad_sq += vdot(grad, grad);
+                       });
     // count = active candidates other than the particle itself, no distance
     // test (solver_dfsph.py:150-160)
     int count = 0;
@@ -691,20 +699,22 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureA
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
     if (gamma != 0.0f || mu != 0.0f) {
-        REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-            if (k == s || !(a.c.sflags[k] & kActive)) return;
-            const float4 xj = a.c.pos[k];
-            const v3 d = vsub(xi, xyz(xj));
-            const float dist = vlen(d);
-            if (gamma != 0.0f)
-                acc = vsub(acc, smul(gamma / m_i * xj.w, muls(d, cubic_w(fmax_(dist, pr), h))));
-            if (mu != 0.0f) {
-                const float4 vj = a.vel[k];
-                float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
-                float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
-                acc = vadd(acc, smul(sv, cubic_grad_r(d, dist, h)));
-            }
-        });
+        REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+            [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vel[k]}; },
+            [&](int k, const NbrXV& nb) {
+                if (k == s || !(nb.flags & kActive)) return;
+                const float4 xj = nb.x;
+                const v3 d = vsub(xi, xyz(xj));
+                const float dist = vlen(d);
+                if (gamma != 0.0f)
+                    acc = vsub(acc, smul(gamma / m_i * xj.w, muls(d, cubic_w(fmax_(dist, pr), h))));
+                if (mu != 0.0f) {
+                    const float4 vj = nb.v;
+                    float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
+                    float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
+                    acc = vadd(acc, smul(sv, cubic_grad_r(d, dist, h)));
+                }
+            }));
     }
     int w = a.world[i];
     w = w > 0 ? w : 0;
@@ -755,12 +765,14 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
             const v3 vi = xyz(a.vstar_s[s]);
             const float h = a.c.h;
             float div_val = 0.0f;
-            REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-                if (!(a.c.sflags[k] & kActive)) return;
-                const float4 xj = a.c.pos[k];
-                v3 grad_w = cubic_grad(vsub(xi, xyz(xj)), h);
-                div_val += xj.w * vdot(vsub(vi, xyz(a.vstar_s[k])), grad_w);
-            });
+            REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+                [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vstar_s[k]}; },
+                [&](int k, const NbrXV& nb) {
+                    if (!(nb.flags & kActive)) return;
+                    const float4 xj = nb.x;
+                    v3 grad_w = cubic_grad(vsub(xi, xyz(xj)), h);
+                    div_val += xj.w * vdot(vsub(vi, xyz(nb.v)), grad_w);
+                }));
             a.d_rho_dt[i] = div_val;
             const bool dyn = (fl & kDynamic) != 0;
             float kap = 0.0f;
@@ -819,12 +831,14 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_correction(DfCorrArgs a)
     const float h = a.c.h;
     const float kri = si.w;
     v3 dv = V(0.0f, 0.0f, 0.0f);
-    REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-        if (!(a.c.sflags[k] & kActive)) return;
-        const float4 xj = a.c.pos[k];
-        float coeff = xj.w * (kri + a.sca[k].w);
-        dv = vadd(dv, smul(coeff, cubic_grad(vsub(xi, xyz(xj)), h)));
-    });
+    REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+        [&](int k) { return NbrXS{a.c.sflags[k], a.c.pos[k], a.sca[k].w}; },
+        [&](int k, const NbrXS& nb) {
+            if (!(nb.flags & kActive)) return;
+            const float4 xj = nb.x;
+            float coeff = xj.w * (kri + nb.s);
+            dv = vadd(dv, smul(coeff, cubic_grad(vsub(xi, xyz(xj)), h)));
+        }));
     const float4 vs4 = a.vstar_s[s];
     const v3 vs = vsub(xyz(vs4), smul(a.dt, dv));
     st3(a.v_star, i, vs);
@@ -941,11 +955,13 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_density(PbfDensityArgs a)
     float rho_temp = 0.0f;
     if (fl & kActive) {
         const float h = c.h;
-        SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-            if (!(c.sflags[k] & kActive)) return;
-            v3 r = vsub(xi, xyz(c.pos[k]));
-            rho_temp += a.wvol[k] * poly6_w(vlen(r), h);
-        });
+        SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+            [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.wvol[k]}; },
+            [&](int k, const NbrXS& nb) {
+                if (!(nb.flags & kActive)) return;
+                v3 r = vsub(xi, xyz(nb.x));
+                rho_temp += nb.s * poly6_w(vlen(r), h);
+            }));
     } else if (c.L.mode == kListBuild) {
         c.L.cnt[s] = 0;
     }
@@ -1006,22 +1022,24 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_nonpressure(PbfNonPressureAr
             const float mu10 = 10.0f * mu;
             v3 acc = V(0.0f, 0.0f, 0.0f);
             if (gamma != 0.0f || mu != 0.0f) {
-                REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-                    if (k == s || !(a.c.sflags[k] & kActive)) return;
-                    const float4 xj = a.c.pos[k];
-                    const v3 d = vsub(xi, xyz(xj));
-                    const float dist = vlen(d);
-                    if (gamma != 0.0f)
-                        acc = vsub(acc, smul(gamma / m_i * xj.w, muls(d, poly6_w(fmax_(dist, pr), h))));
-                    if (mu != 0.0f) {
-                        const float4 vj = a.vel[k];
-                        float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
-                        if (v_current_nei != 0.0f) {
-                            float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
-                            acc = vadd(acc, smul(sv, spiky_grad_r(d, dist, h)));
+                REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+                    [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vel[k]}; },
+                    [&](int k, const NbrXV& nb) {
+                        if (k == s || !(nb.flags & kActive)) return;
+                        const float4 xj = nb.x;
+                        const v3 d = vsub(xi, xyz(xj));
+                        const float dist = vlen(d);
+                        if (gamma != 0.0f)
+                            acc = vsub(acc, smul(gamma / m_i * xj.w, muls(d, poly6_w(fmax_(dist, pr), h))));
+                        if (mu != 0.0f) {
+                            const float4 vj = nb.v;
+                            float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
+                            if (v_current_nei != 0.0f) {
+                                float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
+                                acc = vadd(acc, smul(sv, spiky_grad_r(d, dist, h)));
+                            }
                         }
-                    }
-                });
+                    }));
             }
             st3(a.accel, i, acc);
             const v3 delta_v = smul(a.dt, acc);
@@ -1069,9 +1087,11 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_lambda(PbfLambdaArgs a)
     v3 grad_i = V(0.0f, 0.0f, 0.0f);
     float sum_grad_sq = 0.0f;
     int count = 0;
-    SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-        if (!(c.sflags[k] & kActive)) return;
-        const float4 xj = c.pos[k];
+    SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+        [&](int k) { return NbrX{c.sflags[k], c.pos[k]}; },
+        [&](int k, const NbrX& nb) {
+        if (!(nb.flags & kActive)) return;
+        const float4 xj = nb.x;
         v3 xij = vsub(xi, xyz(xj));
         float r = vlen(xij);
         if (k != s && r < h) count += 1;
@@ -1080,7 +1100,7 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_lambda(PbfLambdaArgs a)
         v3 grad_j = smul(-(mass_j / rho0_i), spiky_grad_r(xij, r, h));
         sum_grad_sq += vdot(grad_j, grad_j);
         grad_i = vsub(grad_i, grad_j);
-    });
+    }));
     sum_grad_sq += vdot(grad_i, grad_i);
     const float c_i = fmax_(rho_i / rho0_i - 1.0f, 0.0f);
     a.rho[i] = rho_i;
@@ -1125,16 +1145,18 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_delta(PbfDeltaArgs a)
             const float lam_i = a.sca[s].x;
             const float h = c.h;
             v3 dp = V(0.0f, 0.0f, 0.0f);
-            REPLAY_NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-                if (k == s || !(c.sflags[k] & kActive)) return;
-                const float4 xj = c.pos[k];
-                v3 xij = vsub(xi, xyz(xj));
-                const float rn = vlen(xij);
-                v3 grad = spiky_grad_r(xij, rn, h);
-                float scorr = 0.0f;
-                if (SCORR) scorr = compute_scorr(rn, h, a.scorr_k, a.scorr_n, a.scorr_w_delta_q);
-                dp = vadd(dp, smul((lam_i + a.sca[k].x + scorr) * xj.w, grad));
-            });
+            REPLAY_NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+                [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.sca[k].x}; },
+                [&](int k, const NbrXS& nb) {
+                    if (k == s || !(nb.flags & kActive)) return;
+                    const float4 xj = nb.x;
+                    v3 xij = vsub(xi, xyz(xj));
+                    const float rn = vlen(xij);
+                    v3 grad = spiky_grad_r(xij, rn, h);
+                    float scorr = 0.0f;
+                    if (SCORR) scorr = compute_scorr(rn, h, a.scorr_k, a.scorr_n, a.scorr_w_delta_q);
+                    dp = vadd(dp, smul((lam_i + nb.s + scorr) * xj.w, grad));
+                }));
             dp = vdivs(dp, rho0_i);
             if (a.max_delta_position > 0.0f) {
                 float dp_len = vlen(dp);
@@ -1177,13 +1199,15 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_xsph(PbfXsphArgs a)
     }
     const float h = c.h;
     v3 corr = V(0.0f, 0.0f, 0.0f);
-    REPLAY_NEIGHBORS(c, s, x.x, x.y, x.z, NoCount{}, [&](int k) {
-        if (k == s || !(c.sflags[k] & kActive)) return;
-        const float4 xj = c.pos[k];
-        const float4 vj = a.vel[k];
-        float w = poly6_w(vlen(vsub(x, xyz(xj))), h);
-        corr = vadd(corr, muls(smul(vj.w, vsub(xyz(vj), vi)), w));
-    });
+    REPLAY_NEIGHBORS(c, s, x.x, x.y, x.z, NoCount{}, phased(
+        [&](int k) { return NbrXV{c.sflags[k], c.pos[k], a.vel[k]}; },
+        [&](int k, const NbrXV& nb) {
+            if (k == s || !(nb.flags & kActive)) return;
+            const float4 xj = nb.x;
+            const float4 vj = nb.v;
+            float w = poly6_w(vlen(vsub(x, xyz(xj))), h);
+            corr = vadd(corr, muls(smul(vj.w, vsub(xyz(vj), vi)), w));
+        }));
     v3 v = vadd(vi, smul(a.xsph_c, corr));
     a.bounds.clamp_damp(x, v, a.damping);
     st3(a.x_out, i, x);
