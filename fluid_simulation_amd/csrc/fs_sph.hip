@@ -74,6 +74,14 @@ struct NbrXVS {  // + a float4 and a scalar
     float s;
 };
 
+struct AdjNbr {   // reverse force pass: flags, x + mass, abar + c, v + m/rho, p/max(rho^2)
+    int flags;
+    float4 x;
+    float4 abar;
+    float4 v;
+    float P;
+};
+
 __device__ __forceinline__ int pass_slot() { return xcd_tile() * kPassBlock + threadIdx.x; }
 template <int BLOCK>
 __device__ __forceinline__ int pass_slot_b() { return xcd_tile() * BLOCK + threadIdx.x; }
@@ -484,16 +492,18 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_force(WadjForceArgs a)
     const float eps2 = 0.01f * (h * h);
     v3 xb = V(0.0f, 0.0f, 0.0f), vb = V(0.0f, 0.0f, 0.0f);
     float Pb = 0.0f, wb = 0.0f, mub = 0.0f;
-    REPLAY_NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, [&](int j) {
-        if (j == s || !(c.sflags[j] & kActive)) return;
-        const float4 xj4 = c.pos[j];
+    REPLAY_NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, phased(
+        [&](int j) { return AdjNbr{c.sflags[j], c.pos[j], a.abar[j], a.vel[j], a.sca[j].z}; },
+        [&](int j, const AdjNbr& nb) {
+        if (j == s || !(nb.flags & kActive)) return;
+        const float4 xj4 = nb.x;
         const v3 r = vsub(xk, xyz(xj4));
         const float d = vlen(r);
         if (!(d / h <= 1.0f)) return;
         const v3 G = cubic_grad_r(r, d, h);
-        const float4 aj4 = a.abar[j];
-        const float4 vj4 = a.vel[j];
-        const float4 sj = a.sca[j];
+        const float4 aj4 = nb.abar;
+        const float4 vj4 = nb.v;
+        const float4 sj = make_float4(0.0f, 0.0f, nb.P, 0.0f);
         const v3 aj = xyz(aj4);
         const float m_j = xj4.w;
         const float P_j = sj.z;
@@ -517,7 +527,7 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_force(WadjForceArgs a)
         vb = vadd(vb, smul((ck_wj * Gak - cj_wk * Gaj) / den, r));
         mub += 10.0f * w_j * sv * Gak;
         wb += -c_j * sv * Gaj;
-    });
+    }));
     float rb = 0.0f;
     const float pb = Pb / fmax_(rho_k * rho_k, 1.0e-6f);
     if (rho_k * rho_k > 1.0e-6f) rb += Pb * (-2.0f * p_k / (rho_k * rho_k * rho_k));
@@ -568,13 +578,15 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_density(WadjDensityArgs a)
         const float wv_k = a.wvol[s];
         const float rb_k = a.rhobar[s];
         const float h = c.h;
-        REPLAY_NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, [&](int j) {
-            if (j == s || !(c.sflags[j] & kActive)) return;
-            const v3 r = vsub(xk, xyz(c.pos[j]));
-            const float d = vlen(r);
-            if (!(d / h <= 1.0f)) return;
-            xb = vadd(xb, smul(rb_k * a.wvol[j] + a.rhobar[j] * wv_k, cubic_grad_r(r, d, h)));
-        });
+        REPLAY_NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, phased(
+            [&](int j) { return NbrXV{c.sflags[j], c.pos[j], make_float4(a.wvol[j], a.rhobar[j], 0.0f, 0.0f)}; },
+            [&](int j, const NbrXV& nb) {
+                if (j == s || !(nb.flags & kActive)) return;
+                const v3 r = vsub(xk, xyz(nb.x));
+                const float d = vlen(r);
+                if (!(d / h <= 1.0f)) return;
+                xb = vadd(xb, smul(rb_k * nb.v.x + nb.v.y * wv_k, cubic_grad_r(r, d, h)));
+            }));
     }
     st3(a.xbar_in, i, V(x0.x + xb.x, x0.y + xb.y, x0.z + xb.z));
     const float4 v0 = a.vbar_s[s];
@@ -1247,15 +1259,17 @@ __global__ __launch_bounds__(BLOCK) void k_separation(SepArgs a)
         const v3 xi = xyz(c.pos[s]);
         const float rest = a.rest_dist, strength = a.strength;
         float count = 0.0f;
-        SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, [&](int k) {
-            if (k == s || !(c.sflags[k] & kActive)) return;
-            const v3 rij = vsub(xi, xyz(c.pos[k]));
-            const float d = vlen(rij);
-            if (d > 1.0e-8f && d < rest) {
-                disp = vadd(disp, muls(vdivs(rij, d), strength * (rest - d)));
-                count += 1.0f;
-            }
-        });
+        SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+            [&](int k) { return NbrX{c.sflags[k], c.pos[k]}; },
+            [&](int k, const NbrX& nb) {
+                if (k == s || !(nb.flags & kActive)) return;
+                const v3 rij = vsub(xi, xyz(nb.x));
+                const float d = vlen(rij);
+                if (d > 1.0e-8f && d < rest) {
+                    disp = vadd(disp, muls(vdivs(rij, d), strength * (rest - d)));
+                    count += 1.0f;
+                }
+            }));
         if (count > 0.0f) disp = vdivs(disp, count);
         else disp = V(0.0f, 0.0f, 0.0f);
         // into-wall components removed near the walls (:233-245)
