@@ -226,50 +226,64 @@ __device__ __forceinline__ uint64_t scan_desc(uint32_t status, int value)
 
 // Single-pass exclusive scan with decoupled look-back.  Tile = blockIdx.x:
 // workgroups are dispatched in index order, so every predecessor of a
-// resident tile has started and will publish.
+// resident tile has started and will publish.  Each wave scans 1024 counters
+// of the tile in 4 rounds of 64 lanes x int4 (every load and store
+// instruction covers 1 KB of contiguous memory), carrying the running total
+// from round to round.
 __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restrict__ in, int* __restrict__ out,
                                                             int m, int ntiles, uint64_t* desc)
 {
-    __shared__ int s_wsum[kScanBlock / 64];
+    constexpr int kWaves = kScanBlock / 64;
+    constexpr int kRounds = kScanItems / 4;
+    constexpr int kWaveItems = kScanTile / kWaves;
+    static_assert(kRounds * 256 == kWaveItems, "scan tiling");
+    __shared__ int s_wsum[kWaves];
     __shared__ int s_excl;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int tile = blockIdx.x;
-    const int base = tile * kScanTile + t * kScanItems;
+    const int wbase = tile * kScanTile + wid * kWaveItems;
 
-    int v[kScanItems];
-    if (base + kScanItems <= m) {
-        const int4* p = reinterpret_cast<const int4*>(in + base);
+    int4 q[kRounds];
 #pragma unroll
-        for (int j = 0; j < kScanItems / 4; ++j) {
-            int4 q = p[j];
-            v[4 * j] = q.x;
-            v[4 * j + 1] = q.y;
-            v[4 * j + 2] = q.z;
-            v[4 * j + 3] = q.w;
+    for (int j = 0; j < kRounds; ++j) {
+        const int b = wbase + j * 256 + lane * 4;
+        if (b + 4 <= m) {
+            q[j] = *reinterpret_cast<const int4*>(in + b);
+        } else {
+            q[j].x = (b < m) ? in[b] : 0;
+            q[j].y = (b + 1 < m) ? in[b + 1] : 0;
+            q[j].z = (b + 2 < m) ? in[b + 2] : 0;
+            q[j].w = 0;
         }
-    } else {
-#pragma unroll
-        for (int j = 0; j < kScanItems; ++j) v[j] = (base + j < m) ? in[base + j] : 0;
     }
+    int v[kScanItems];   // per round: the lane's 4 counters, inclusive-scanned
+    int lex[kRounds];    // exclusive offset of the lane's group within the wave
+    int wsum = 0;
 #pragma unroll
-    for (int j = 1; j < kScanItems; ++j) v[j] += v[j - 1];
-    const int tsum = v[kScanItems - 1];
-    int x = tsum;
+    for (int j = 0; j < kRounds; ++j) {
+        v[4 * j] = q[j].x;
+        v[4 * j + 1] = v[4 * j] + q[j].y;
+        v[4 * j + 2] = v[4 * j + 1] + q[j].z;
+        v[4 * j + 3] = v[4 * j + 2] + q[j].w;
+        const int tsum = v[4 * j + 3];
+        int x = tsum;
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        lex[j] = wsum + x - tsum;
+        wsum += __shfl(x, 63, 64);
     }
-    if (lane == 63) s_wsum[wid] = x;
+    if (lane == 0) s_wsum[wid] = wsum;
     __syncthreads();
     int wprefix = 0, agg = 0;
 #pragma unroll
-    for (int w = 0; w < kScanBlock / 64; ++w) {
+    for (int w = 0; w < kWaves; ++w) {
         int ws = s_wsum[w];
         if (w < wid) wprefix += ws;
         agg += ws;
     }
-    const int thread_excl = wprefix + x - tsum;
 
     if (wid == 0) {
         int excl = 0;
@@ -304,22 +318,23 @@ __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restric
         if (lane == 0) s_excl = excl;
     }
     __syncthreads();
-    const int excl = s_excl + thread_excl;
-    if (base + kScanItems <= m) {
-        int4* p = reinterpret_cast<int4*>(out + base);
+    const int wexcl = s_excl + wprefix;
 #pragma unroll
-        for (int j = 0; j < kScanItems / 4; ++j) {
-            int4 q;
-            q.x = excl + (4 * j == 0 ? 0 : v[4 * j - 1]);
-            q.y = excl + v[4 * j];
-            q.z = excl + v[4 * j + 1];
-            q.w = excl + v[4 * j + 2];
-            p[j] = q;
+    for (int j = 0; j < kRounds; ++j) {
+        const int b = wbase + j * 256 + lane * 4;
+        const int e = wexcl + lex[j];
+        int4 o;
+        o.x = e;
+        o.y = e + v[4 * j];
+        o.z = e + v[4 * j + 1];
+        o.w = e + v[4 * j + 2];
+        if (b + 4 <= m) {
+            *reinterpret_cast<int4*>(out + b) = o;
+        } else {
+            if (b < m) out[b] = o.x;
+            if (b + 1 < m) out[b + 1] = o.y;
+            if (b + 2 < m) out[b + 2] = o.z;
         }
-    } else {
-#pragma unroll
-        for (int j = 0; j < kScanItems; ++j)
-            if (base + j < m) out[base + j] = excl + (j == 0 ? 0 : v[j - 1]);
     }
     if (tile == ntiles - 1 && t == 0) out[m] = s_excl + agg;
 }
