@@ -24,7 +24,8 @@ namespace fs {
 
 constexpr int kPassBlock = 256;
 constexpr int kScanBlock = 64;  // workgroup of the untuned scanning passes (DFSPH, PBF)
-constexpr int kScanCap = 24;    // LDS survivor list of a scanning pass
+constexpr int kScanCap = 48;    // LDS survivor list of a scanning pass (DFSPH, PBF)
+constexpr int kSepCap = 24;     // separation: few survivors, a shorter list keeps occupancy
 constexpr int kReplayCap = 16;  // ... of a list-replaying pass (only its overflow fallback scans)
 constexpr float kCutSlack = 1.0001f;   // candidates with d2 > h^2*slack have q > 1
 
@@ -1299,8 +1300,8 @@ struct Variant {
     int cap;
 };
 // variant 0 is the default
-constexpr Variant kScanVariants[] = {{64, 24},  {64, 32},  {64, 48},  {128, 24}, {128, 32}, {128, 16},
-                                     {256, 32}, {256, 24}, {256, 16}};
+constexpr Variant kScanVariants[] = {{64, 48},  {64, 24},  {64, 32},  {64, 64},  {128, 24}, {128, 32},
+                                     {128, 16}, {256, 32}, {256, 24}, {256, 16}};
 constexpr Variant kReplayVariants[] = {{64, 16}, {128, 16}, {256, 16}, {512, 16}};
 constexpr int kNumScan = sizeof(kScanVariants) / sizeof(kScanVariants[0]);
 constexpr int kNumReplay = sizeof(kReplayVariants) / sizeof(kReplayVariants[0]);
@@ -1967,7 +1968,7 @@ int32_t fs_particle_separation(fs_grid* g, fs_work* w, const fs_separation_param
         if (rc) return rc;
         a.c = make_common(g, w, reinterpret_cast<const float4*>(w->pos), prm->query_radius, kListNone);
         a.c.cut2 = prm->rest_dist * prm->rest_dist * kCutSlack;
-        FS_LAUNCH((k_separation<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s,
+        FS_LAUNCH((k_separation<kScanBlock, kSepCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s,
                   a);   // :1037-1066
     }
     return FS_OK;
