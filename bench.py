@@ -178,13 +178,13 @@ def bench_solver(method, scene, steps, warmup, tune=True):
     return out, prof
 
 
-def cpu_baseline(scene, substeps):
+def cpu_baseline(scene, substeps, threads=None):
     import numpy as np
 
     from oracle import oracle as O
     from fluid_simulation_amd.examples.dam_break.simulation import material, solver_config
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = threads or min(os.cpu_count() or 1, 16)
     O.set_threads(threads)
     cfg = solver_config("wcsph", scene)
     mat = material("wcsph", scene)
@@ -204,6 +204,25 @@ def cpu_baseline(scene, substeps):
     return {"value": n * substeps / el, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
             "sample": f"{substeps} WCSPH substeps of the {n}-particle dam break, FP32 C oracle, OpenMP "
                       f"{threads} threads, {el:.1f} s"}
+
+
+def cpu_baseline_detail(scene):
+    """The oracle's WCSPH substep rate single-threaded (the analogue of the
+    reference's CPU path, one host thread per launch) and on all host cores
+    (up to 16), at the headline size and at the 10,164-particle column."""
+    from fluid_simulation_amd.adjoint import column_scene
+    from fluid_simulation_amd.examples.dam_break.scene import DEFAULT_SPEC, ParticleScene
+
+    col = column_scene(10000)
+    s = col.spacing
+    column = ParticleScene(DEFAULT_SPEC, col.positions, 1000.0 * s ** 3, 0.5 * s, s ** 3, s)
+    threads = min(os.cpu_count() or 1, 16)
+    out = {}
+    for name, sc, many, one in (("dam_break_401856", scene, 20, 2), ("column_10164", column, 200, 20)):
+        out[name] = {"threads_%d" % threads: cpu_baseline(sc, many)["value"],
+                     "threads_1": cpu_baseline(sc, one, threads=1)["value"]}
+    out["unit"] = "particle-substeps/s"
+    return out
 
 
 def copy_calibration():
@@ -690,10 +709,12 @@ def main():
         extras.update(frames(count=2))
         print("# APIC particle separation: 401,856 particles, 2 iterations", file=sys.stderr, flush=True)
         extras.update(separation(cpu=not args.no_cpu_baseline))
-    cpu = None
+    cpu = cpu_detail = None
     if not args.no_cpu_baseline:
         print("# cpu baseline (oracle)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(scene, args.cpu_substeps)
+        if args.particles == 400000:
+            cpu_detail = cpu_baseline_detail(scene)
     value = headline["particle_substeps_per_s"]
     line = {
         "metric": "particle-substeps/s @402k dam-break, % HBM peak; batched & adjoint vs plain",
@@ -713,6 +734,7 @@ def main():
         "hbm_frac_substep": headline["hbm_frac"],
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_detail": cpu_detail,
         "solvers": results,
         **extras,
         "device": torch.cuda.get_device_name(0),
