@@ -158,6 +158,38 @@ SIGNATURES = {
                                 POINTER(FsDfsphIO), c_void_p]),
     "fs_pbf_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
                               POINTER(FsPbfIO), c_void_p]),
+    "fs_wcsph_density_eos": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsWcsphParams),
+                                       POINTER(FsWcsphIO), c_void_p]),
+    "fs_wcsph_force_integrate": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel),
+                                           POINTER(FsWcsphParams), POINTER(FsWcsphIO), c_void_p]),
+    "fs_dfsph_density_factor": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsDfsphParams),
+                                          POINTER(FsDfsphIO), c_int32, c_void_p]),
+    "fs_dfsph_nonpressure_predict": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel),
+                                               POINTER(FsDfsphParams), POINTER(FsDfsphIO), c_void_p]),
+    "fs_dfsph_rate_kappa": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsDfsphParams),
+                                      POINTER(FsDfsphIO), c_int32, c_void_p]),
+    "fs_dfsph_residual": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsDfsphParams),
+                                    POINTER(FsDfsphIO), c_int32, c_int32, c_void_p]),
+    "fs_dfsph_correction": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsDfsphParams),
+                                      POINTER(FsDfsphIO), c_int32, c_int32, c_void_p]),
+    "fs_dfsph_integrate": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsDfsphParams),
+                                     POINTER(FsDfsphIO), c_void_p]),
+    "fs_dfsph_finalize": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsDfsphParams),
+                                    POINTER(FsDfsphIO), c_void_p]),
+    "fs_pbf_predict": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
+                                 POINTER(FsPbfIO), c_void_p]),
+    "fs_pbf_density": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
+                                 POINTER(FsPbfIO), c_void_p]),
+    "fs_pbf_nonpressure": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
+                                     POINTER(FsPbfIO), c_void_p]),
+    "fs_pbf_lambda": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
+                                POINTER(FsPbfIO), c_int32, c_void_p]),
+    "fs_pbf_delta": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
+                               POINTER(FsPbfIO), c_int32, c_void_p]),
+    "fs_pbf_final_density": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
+                                       POINTER(FsPbfIO), c_void_p]),
+    "fs_pbf_xsph_finalize": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), POINTER(FsPbfParams),
+                                       POINTER(FsPbfIO), c_void_p]),
     "fs_particle_separation": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsSeparationParams), c_void_p,
                                          c_void_p, c_void_p, c_int32, c_void_p]),
     "fs_batch_wcsph_step": (c_int32, [POINTER(FsGrid), POINTER(FsWork), POINTER(FsModel), c_void_p, c_int32,

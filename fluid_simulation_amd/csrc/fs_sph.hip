@@ -1478,19 +1478,18 @@ namespace fs {
 
 // grid build + density/EOS pass (records neighbour lists) + force/integrate
 // pass (replays them): SolverWCSPH.step  solver_wcsph.py:431-535
-static int32_t wcsph_forward(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
-                             const fs_wcsph_io* io, hipStream_t s)
+// build (solver_wcsph.py:456) + compute_density_pressure (:458-474); the
+// pass records the neighbour lists
+static int32_t wcsph_density_eos(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
+                                 const fs_wcsph_io* io, hipStream_t s)
 {
-    int32_t rc;
     const int n = m->n;
     const float h = prm->smoothing_length;
     PreOp pre;
     pre.x_in = io->x_in;
-    rc = build_grid(g, n, h, pre, gather_for(m, w, io->x_in, io->v_in, reinterpret_cast<float4*>(w->vel)), s);
+    int32_t rc = build_grid(g, n, h, pre, gather_for(m, w, io->x_in, io->v_in, reinterpret_cast<float4*>(w->vel)), s);
     if (rc) return rc;
-    const int nb = blocks_for(n, kPassBlock);
     const float4* pos = reinterpret_cast<const float4*>(w->pos);
-
     WcsphDensityArgs da;
     da.c = make_common(g, w, pos, h, kListBuild);
     da.wvol = w->wvol;
@@ -1505,9 +1504,17 @@ static int32_t wcsph_forward(fs_grid* g, fs_work* w, const fs_model* m, const fs
     da.vel = reinterpret_cast<float4*>(w->vel);
     da.scenes = nullptr;
     da.sscene = nullptr;
-    rc = launch_density<false>(tuned(0), n, s, da);
-    if (rc) return rc;
+    return launch_density<false>(tuned(0), n, s, da);
+}
 
+// acceleration_wcsph + integrate_velocity + integrate_positions + apply_bounds
+// (solver_wcsph.py:475-535), replaying the density pass's lists
+static int32_t wcsph_force_integrate(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
+                                     const fs_wcsph_io* io, hipStream_t s)
+{
+    const int n = m->n;
+    const float h = prm->smoothing_length;
+    const float4* pos = reinterpret_cast<const float4*>(w->pos);
     WcsphForceArgs fa;
     fa.c = make_common(g, w, pos, h, kListUse);
     fa.vel = reinterpret_cast<const float4*>(w->vel);
@@ -1529,6 +1536,14 @@ static int32_t wcsph_forward(fs_grid* g, fs_work* w, const fs_model* m, const fs
     fa.scenes = nullptr;
     fa.sscene = nullptr;
     return launch_force<false>(tuned(1), n, s, fa);
+}
+
+static int32_t wcsph_forward(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
+                             const fs_wcsph_io* io, hipStream_t s)
+{
+    int32_t rc = wcsph_density_eos(g, w, m, prm, io, s);
+    if (rc) return rc;
+    return wcsph_force_integrate(g, w, m, prm, io, s);
 }
 
 static int32_t check_wcsph(const fs_grid* g, const fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
@@ -1607,6 +1622,29 @@ int32_t fs_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_
     if (rc) return rc;
     if (m->n == 0) return FS_OK;
     return wcsph_forward(g, w, m, prm, io, (hipStream_t)stream);
+}
+
+int32_t fs_wcsph_density_eos(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
+                             const fs_wcsph_io* io, fs_stream_t stream)
+{
+    int32_t rc = check_wcsph(g, w, m, prm, io, "fs_wcsph_density_eos");
+    if (rc) return rc;
+    if (m->n == 0) return FS_OK;
+    return wcsph_density_eos(g, w, m, prm, io, (hipStream_t)stream);
+}
+
+int32_t fs_wcsph_force_integrate(fs_grid* g, fs_work* w, const fs_model* m, const fs_wcsph_params* prm,
+                                 const fs_wcsph_io* io, fs_stream_t stream)
+{
+    int32_t rc = check_wcsph(g, w, m, prm, io, "fs_wcsph_force_integrate");
+    if (rc) return rc;
+    if (m->n == 0) return FS_OK;
+    if (g->num_points != m->n) {
+        set_error("fs_wcsph_force_integrate: the grid does not hold this model's particles (run fs_wcsph_density_eos "
+                  "first)");
+        return FS_ERR_ARG;
+    }
+    return wcsph_force_integrate(g, w, m, prm, io, (hipStream_t)stream);
 }
 
 size_t fs_wcsph_adjoint_bytes(int32_t capacity)
@@ -1769,107 +1807,169 @@ int32_t fs_batch_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_
     return launch_force<true>(tuned(3), n, s, fa);
 }
 
-int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm, const fs_dfsph_io* io,
-                      fs_stream_t stream)
+// One DFSPH substep as a sequence of passes sharing the grid and workspace.
+struct DfPlan {
+    fs_grid* g;
+    fs_work* w;
+    const fs_model* m;
+    const fs_dfsph_params* prm;
+    const fs_dfsph_io* io;
+    hipStream_t s;
+    int n, nb;
+    float h, dt;
+    float4* vstar_s;
+    float4* sca;
+    float4* vel;
+    const float4* pos;
+};
+
+static int32_t df_plan(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm, const fs_dfsph_io* io,
+                       fs_stream_t stream, const char* what, DfPlan& p)
 {
     int32_t rc = check_common(g, w, m);
     if (rc) return rc;
     if (!prm || !io) {
-        set_error("fs_dfsph_step: null params or io");
+        set_error("%s: null params or io", what);
         return FS_ERR_ARG;
     }
-    const int n = m->n;
-    if (n == 0) return FS_OK;
-    if (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->factor || !io->rho_star ||
-        !io->d_rho_dt || !io->kappa || !io->kappa_v || !io->neighbor_count || !io->v_star || !io->accel ||
-        (prm->warm_start && (!io->kappa_in || !io->kappa_v_in))) {
-        set_error("fs_dfsph_step: missing arrays");
+    if (m->n > 0 && (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->factor ||
+                     !io->rho_star || !io->d_rho_dt || !io->kappa || !io->kappa_v || !io->neighbor_count ||
+                     !io->v_star || !io->accel || (prm->warm_start && (!io->kappa_in || !io->kappa_v_in)))) {
+        set_error("%s: missing arrays", what);
         return FS_ERR_ARG;
     }
-    hipStream_t s = (hipStream_t)stream;
-    const float h = prm->smoothing_length;
-    const float dt = prm->dt;
-    const int nb = blocks_for(n, kPassBlock);
-    float4* vstar_s = reinterpret_cast<float4*>(w->pos_alt);
-    float4* sca = reinterpret_cast<float4*>(w->sca);
-    float4* vel = reinterpret_cast<float4*>(w->vel);
+    p.g = g;
+    p.w = w;
+    p.m = m;
+    p.prm = prm;
+    p.io = io;
+    p.s = (hipStream_t)stream;
+    p.n = m->n;
+    p.nb = blocks_for(m->n, kPassBlock);
+    p.h = prm->smoothing_length;
+    p.dt = prm->dt;
+    p.vstar_s = reinterpret_cast<float4*>(w->pos_alt);
+    p.sca = reinterpret_cast<float4*>(w->sca);
+    p.vel = reinterpret_cast<float4*>(w->vel);
+    p.pos = reinterpret_cast<const float4*>(w->pos);
+    return FS_OK;
+}
 
-    // build on x_in (solver_dfsph.py:624)
-    PreOp pre;
-    pre.x_in = io->x_in;
-    pre.control_reset = w->control;
-    GatherOp ga1 = gather_for(m, w, io->x_in, io->v_in, vel);
-    ga1.any_inactive = w->control + kAnyInactive;
-    rc = build_grid(g, n, h, pre, ga1, s);
-    if (rc) return rc;
-    const float4* pos = reinterpret_cast<const float4*>(w->pos);
-
+// stage 0: build on x_in (solver_dfsph.py:624) + compute_density +
+// compute_factor_dfsph (:626-653); stage 1: the same passes on the grid the
+// integrate step rebuilt at x_out (:785-812)
+static int32_t df_density_factor(const DfPlan& p, bool after_integrate)
+{
+    const fs_model* m = p.m;
+    const fs_dfsph_io* io = p.io;
+    fs_work* w = p.w;
+    if (!after_integrate) {
+        PreOp pre;
+        pre.x_in = io->x_in;
+        pre.control_reset = w->control;
+        GatherOp ga = gather_for(m, w, io->x_in, io->v_in, p.vel);
+        ga.any_inactive = w->control + kAnyInactive;
+        int32_t rc = build_grid(p.g, p.n, p.h, pre, ga, p.s);
+        if (rc) return rc;
+    }
     DfDensityArgs dfa;
-    dfa.c = make_common(g, w, pos, h, kListBuild);
+    dfa.c = make_common(p.g, w, p.pos, p.h, kListBuild);
     dfa.any_inactive = w->control + kAnyInactive;
     dfa.wvol = w->wvol;
-    dfa.min_neighbors = prm->min_neighbors_for_factor;
-    dfa.kappa_warm = prm->warm_start ? io->kappa_in : nullptr;
-    dfa.kappa_commit = io->kappa;
+    dfa.min_neighbors = p.prm->min_neighbors_for_factor;
+    dfa.kappa_warm = p.prm->warm_start ? (after_integrate ? io->kappa_v_in : io->kappa_in) : nullptr;
+    dfa.kappa_commit = after_integrate ? io->kappa_v : io->kappa;
     dfa.rho = io->rho;
     dfa.factor = io->factor;
     dfa.neighbor_count = io->neighbor_count;
-    dfa.sca = sca;
-    dfa.vel = vel;
-    FS_LAUNCH((k_dfsph_density_factor<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, dfa);
+    dfa.sca = p.sca;
+    dfa.vel = p.vel;
+    FS_LAUNCH((k_dfsph_density_factor<kScanBlock, kScanCap>), dim3(blocks_for(p.n, kScanBlock)), dim3(kScanBlock), 0,
+              p.s, dfa);
+    return FS_OK;
+}
 
-    const PassCommon use = make_common(g, w, pos, h, kListUse);
+// acceleration_non_pressure + predict_velocity  (solver_dfsph.py:654-688)
+static int32_t df_nonpressure_predict(const DfPlan& p)
+{
     DfNonPressureArgs npa;
-    npa.c = use;
-    npa.vel = vel;
-    npa.surface_tension = m->surface_tension;
-    npa.viscosity = m->viscosity;
-    npa.gravity = m->gravity;
-    npa.world = m->world;
-    npa.particle_radius = prm->particle_radius;
-    npa.dt = dt;
-    npa.accel = io->accel;
-    npa.v_star = io->v_star;
-    npa.vstar_s = vstar_s;
-    LAUNCH_PASS(k_dfsph_nonpressure<kReplayCap>, nb, s, npa);
+    npa.c = make_common(p.g, p.w, p.pos, p.h, kListUse);
+    npa.vel = p.vel;
+    npa.surface_tension = p.m->surface_tension;
+    npa.viscosity = p.m->viscosity;
+    npa.gravity = p.m->gravity;
+    npa.world = p.m->world;
+    npa.particle_radius = p.prm->particle_radius;
+    npa.dt = p.dt;
+    npa.accel = p.io->accel;
+    npa.v_star = p.io->v_star;
+    npa.vstar_s = p.vstar_s;
+    LAUNCH_PASS(k_dfsph_nonpressure<kReplayCap>, p.nb, p.s, npa);
+    return FS_OK;
+}
 
+// apply_constant_density_correction (loop 0) or apply_divergence_free_correction
+// (loop 1)  (solver_dfsph.py:309-337, 375-403).  The warm-start form
+// (:689-707, :814-832) applies the warm-started kappa and neither stops nor
+// counts; the loop form returns at once once the loop has stopped.
+static int32_t df_correction(const DfPlan& p, int loop, bool warm)
+{
     DfCorrArgs ca;
-    ca.c = use;
-    ca.sca = sca;
-    ca.dt = dt;
-    ca.v_star = io->v_star;
-    ca.vstar_s = vstar_s;
-    ca.control = w->control;
-    if (prm->warm_start) {   // solver_dfsph.py:689-707
+    ca.c = make_common(p.g, p.w, p.pos, p.h, kListUse);
+    ca.sca = p.sca;
+    ca.dt = p.dt;
+    ca.v_star = p.io->v_star;
+    ca.vstar_s = p.vstar_s;
+    ca.control = p.w->control;
+    if (warm) {
         ca.stop_idx = -1;
         ca.corr_idx = -1;
         ca.kappa_commit = nullptr;
-        LAUNCH_PASS(k_dfsph_correction<kReplayCap>, nb, s, ca);
+    } else {
+        ca.stop_idx = loop == 0 ? kStopRho : kStopVel;
+        ca.corr_idx = loop == 0 ? kCorrRho : kCorrVel;
+        ca.kappa_commit = loop == 0 ? p.io->kappa : p.io->kappa_v;
     }
+    LAUNCH_PASS(k_dfsph_correction<kReplayCap>, p.nb, p.s, ca);
+    return FS_OK;
+}
 
+// compute_density_change_rate + predict_density_star + compute_kappa_density
+// (loop 0) or compute_kappa_divergence (loop 1), with the residual's
+// per-block partial sums  (solver_dfsph.py:712-750, 835-866)
+static int32_t df_rate_kappa(const DfPlan& p, int loop)
+{
     DfRateArgs ra;
-    ra.c = use;
-    ra.vstar_s = vstar_s;
-    ra.rest_density = m->rest_density;
-    ra.dt = dt;
-    ra.d_rho_dt = io->d_rho_dt;
-    ra.rho_star = io->rho_star;
-    ra.sca = sca;
-    ra.partials = w->partials;
-    ra.control = w->control;
-    ra.density_loop = 1;
-    ra.max_kappa = prm->max_kappa;
-    ca.stop_idx = kStopRho;
-    ca.corr_idx = kCorrRho;
-    ca.kappa_commit = io->kappa;
-    for (int it = 0; it < prm->max_rho_iterations; ++it) {   // solver_dfsph.py:711-767
-        LAUNCH_PASS(k_dfsph_rate<kReplayCap>, nb, s, ra);
-        FS_LAUNCH(k_dfsph_residual, dim3(1), dim3(kPassBlock), 0, s, w->partials, nb, w->control, kStopRho, kItRho,
-                  it, prm->eta_density);
-        LAUNCH_PASS(k_dfsph_correction<kReplayCap>, nb, s, ca);
-    }
+    ra.c = make_common(p.g, p.w, p.pos, p.h, kListUse);
+    ra.vstar_s = p.vstar_s;
+    ra.rest_density = p.m->rest_density;
+    ra.dt = p.dt;
+    ra.d_rho_dt = p.io->d_rho_dt;
+    ra.rho_star = p.io->rho_star;
+    ra.sca = p.sca;
+    ra.partials = p.w->partials;
+    ra.control = p.w->control;
+    ra.density_loop = loop == 0 ? 1 : 0;
+    ra.max_kappa = loop == 0 ? p.prm->max_kappa : p.prm->max_kappa_v;
+    LAUNCH_PASS(k_dfsph_rate<kReplayCap>, p.nb, p.s, ra);
+    return FS_OK;
+}
 
-    // integrate_positions fused into the second build's hash (solver_dfsph.py:769-783)
+// _density_residual / _divergence_residual and the loop's break rule
+// (solver_dfsph.py:574-596, 733-735, 849-851), decided on the device
+static int32_t df_residual(const DfPlan& p, int loop, int iteration)
+{
+    FS_LAUNCH(k_dfsph_residual, dim3(1), dim3(kPassBlock), 0, p.s, p.w->partials, p.nb, p.w->control,
+              loop == 0 ? kStopRho : kStopVel, loop == 0 ? kItRho : kItVel, iteration,
+              loop == 0 ? p.prm->eta_density : p.prm->eta_divergence);
+    return FS_OK;
+}
+
+// integrate_positions fused into the second build's hash (solver_dfsph.py:769-783)
+static int32_t df_integrate(const DfPlan& p)
+{
+    const fs_model* m = p.m;
+    const fs_dfsph_io* io = p.io;
     PreOp pre2;
     pre2.mode = kPreIntegrate;
     pre2.x_in = io->x_in;
@@ -1877,50 +1977,87 @@ int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_
     pre2.x_out = io->x_out;
     pre2.flags = m->flags;
     pre2.mask = m->mask;
-    pre2.dt = dt;
-    GatherOp ga2 = gather_for(m, w, io->x_out, io->v_star, vstar_s);
-    ga2.any_inactive = w->control + kAnyInactive;   // same particles: sets the same flag again
-    rc = build_grid(g, n, h, pre2, ga2, s);
-    if (rc) return rc;
-    dfa.c = make_common(g, w, pos, h, kListBuild);
-    dfa.kappa_warm = prm->warm_start ? io->kappa_v_in : nullptr;
-    dfa.kappa_commit = io->kappa_v;
-    FS_LAUNCH((k_dfsph_density_factor<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, dfa);   // solver_dfsph.py:785-812
+    pre2.dt = p.dt;
+    GatherOp ga2 = gather_for(m, p.w, io->x_out, io->v_star, p.vstar_s);
+    ga2.any_inactive = p.w->control + kAnyInactive;   // same particles: sets the same flag again
+    return build_grid(p.g, p.n, p.h, pre2, ga2, p.s);
+}
 
-    const PassCommon use2 = make_common(g, w, pos, h, kListUse);
-    ca.c = use2;
-    if (prm->warm_start) {   // solver_dfsph.py:814-832
-        ca.stop_idx = -1;
-        ca.corr_idx = -1;
-        ca.kappa_commit = nullptr;
-        LAUNCH_PASS(k_dfsph_correction<kReplayCap>, nb, s, ca);
-    }
-    ra.c = use2;
-    ra.density_loop = 0;
-    ra.max_kappa = prm->max_kappa_v;
-    ca.stop_idx = kStopVel;
-    ca.corr_idx = kCorrVel;
-    ca.kappa_commit = io->kappa_v;
-    for (int it = 0; it < prm->max_vel_iterations; ++it) {   // solver_dfsph.py:834-883
-        LAUNCH_PASS(k_dfsph_rate<kReplayCap>, nb, s, ra);
-        FS_LAUNCH(k_dfsph_residual, dim3(1), dim3(kPassBlock), 0, s, w->partials, nb, w->control, kStopVel, kItVel,
-                  it, prm->eta_divergence);
-        LAUNCH_PASS(k_dfsph_correction<kReplayCap>, nb, s, ca);
-    }
-
+// apply_bounds (DFSPH variant) and the outputs  (solver_dfsph.py:886-898)
+static int32_t df_finalize(const DfPlan& p)
+{
     DfFinalArgs fa;
-    fa.n = n;
-    fa.flags = m->flags;
-    fa.mask = m->mask;
-    fa.v_star = io->v_star;
-    fa.x_out = io->x_out;
-    fa.v_out = io->v_out;
-    for (int d = 0; d < 3; ++d) fa.bound[d] = prm->bound[d];
-    fa.damping = prm->boundary_damping;
-    fa.control = w->control;
-    fa.iterations = io->iterations;
-    FS_LAUNCH(k_dfsph_finalize, dim3(blocks_for(n, kElemBlock)), dim3(kElemBlock), 0, s, fa);
+    fa.n = p.n;
+    fa.flags = p.m->flags;
+    fa.mask = p.m->mask;
+    fa.v_star = p.io->v_star;
+    fa.x_out = p.io->x_out;
+    fa.v_out = p.io->v_out;
+    for (int d = 0; d < 3; ++d) fa.bound[d] = p.prm->bound[d];
+    fa.damping = p.prm->boundary_damping;
+    fa.control = p.w->control;
+    fa.iterations = p.io->iterations;
+    FS_LAUNCH(k_dfsph_finalize, dim3(blocks_for(p.n, kElemBlock)), dim3(kElemBlock), 0, p.s, fa);
     return FS_OK;
+}
+
+int32_t fs_dfsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm, const fs_dfsph_io* io,
+                      fs_stream_t stream)
+{
+    DfPlan p;
+    int32_t rc = df_plan(g, w, m, prm, io, stream, "fs_dfsph_step", p);
+    if (rc) return rc;
+    if (p.n == 0) return FS_OK;
+#define DF_TRY(call)    \
+    do {                \
+        rc = (call);    \
+        if (rc) return rc; \
+    } while (0)
+    DF_TRY(df_density_factor(p, false));
+    DF_TRY(df_nonpressure_predict(p));
+    if (prm->warm_start) DF_TRY(df_correction(p, 0, true));
+    for (int it = 0; it < prm->max_rho_iterations; ++it) {   // solver_dfsph.py:711-767
+        DF_TRY(df_rate_kappa(p, 0));
+        DF_TRY(df_residual(p, 0, it));
+        DF_TRY(df_correction(p, 0, false));
+    }
+    DF_TRY(df_integrate(p));
+    DF_TRY(df_density_factor(p, true));
+    if (prm->warm_start) DF_TRY(df_correction(p, 1, true));
+    for (int it = 0; it < prm->max_vel_iterations; ++it) {   // solver_dfsph.py:834-883
+        DF_TRY(df_rate_kappa(p, 1));
+        DF_TRY(df_residual(p, 1, it));
+        DF_TRY(df_correction(p, 1, false));
+    }
+    return df_finalize(p);
+#undef DF_TRY
+}
+
+// the passes of fs_dfsph_step, one call each
+static int32_t df_pass(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm, const fs_dfsph_io* io,
+                       fs_stream_t stream, const char* what, int which, int loop, int iteration)
+{
+    DfPlan p;
+    int32_t rc = df_plan(g, w, m, prm, io, stream, what, p);
+    if (rc) return rc;
+    if (p.n == 0) return FS_OK;
+    if ((which != 0 && which != 6) && g->num_points != m->n) {
+        set_error("%s: the grid does not hold this model's particles (run fs_dfsph_density_factor first)", what);
+        return FS_ERR_ARG;
+    }
+    if ((which == 2 || which == 3 || which == 4) && loop != 0 && loop != 1) {
+        set_error("%s: loop must be 0 (density) or 1 (divergence)", what);
+        return FS_ERR_ARG;
+    }
+    switch (which) {
+    case 0: return df_density_factor(p, loop != 0);
+    case 1: return df_nonpressure_predict(p);
+    case 2: return df_rate_kappa(p, loop);
+    case 3: return df_residual(p, loop, iteration);
+    case 4: return df_correction(p, loop, iteration < 0);
+    case 5: return df_integrate(p);
+    default: return df_finalize(p);
+    }
 }
 
 int32_t fs_particle_separation(fs_grid* g, fs_work* w, const fs_separation_params* prm, float* positions,
@@ -1974,142 +2111,324 @@ int32_t fs_particle_separation(fs_grid* g, fs_work* w, const fs_separation_param
     return FS_OK;
 }
 
-int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,
-                    fs_stream_t stream)
+// One PBF substep as a sequence of passes sharing the grid and workspace.
+// Predicted positions ping-pong between the workspace's two sorted position
+// streams; which one holds them follows from the iteration alone.
+struct PbfPlan {
+    fs_grid* g;
+    fs_work* w;
+    const fs_model* m;
+    const fs_pbf_params* prm;
+    const fs_pbf_io* io;
+    hipStream_t s;
+    int n, nb;
+    float h, dt;
+    float4* sca;
+    float4* vel;
+    float4* bufs[2];
+    PbfBounds bounds;
+};
+
+static int32_t pbf_plan(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,
+                        fs_stream_t stream, const char* what, PbfPlan& p)
 {
     int32_t rc = check_common(g, w, m);
     if (rc) return rc;
     if (!prm || !io) {
-        set_error("fs_pbf_step: null params or io");
+        set_error("%s: null params or io", what);
         return FS_ERR_ARG;
     }
-    const int n = m->n;
-    if (n == 0) return FS_OK;
-    if (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->pbf_lambda || !io->neighbor_count ||
-        !io->delta_p || !io->accel) {
-        set_error("fs_pbf_step: missing arrays");
+    if (m->n > 0 && (!io->x_in || !io->v_in || !io->x_out || !io->v_out || !io->rho || !io->pbf_lambda ||
+                     !io->neighbor_count || !io->delta_p || !io->accel)) {
+        set_error("%s: missing arrays", what);
         return FS_ERR_ARG;
     }
-    hipStream_t s = (hipStream_t)stream;
-    const float h = prm->smoothing_length;
-    const float dt = prm->dt;
-    const int nb = blocks_for(n, kPassBlock);
-    float4* sca = reinterpret_cast<float4*>(w->sca);
-    float4* vel = reinterpret_cast<float4*>(w->vel);
-    float4* bufs[2] = {reinterpret_cast<float4*>(w->pos), reinterpret_cast<float4*>(w->pos_alt)};
-    int cur = 0;
-    PbfBounds bounds;
-    for (int d = 0; d < 3; ++d) bounds.bound[d] = prm->bound[d];
-    bounds.particle_radius = prm->particle_radius;
-    bounds.boundary_epsilon = prm->boundary_epsilon;
+    p.g = g;
+    p.w = w;
+    p.m = m;
+    p.prm = prm;
+    p.io = io;
+    p.s = (hipStream_t)stream;
+    p.n = m->n;
+    p.nb = blocks_for(m->n, kPassBlock);
+    p.h = prm->smoothing_length;
+    p.dt = prm->dt;
+    p.sca = reinterpret_cast<float4*>(w->sca);
+    p.vel = reinterpret_cast<float4*>(w->vel);
+    p.bufs[0] = reinterpret_cast<float4*>(w->pos);
+    p.bufs[1] = reinterpret_cast<float4*>(w->pos_alt);
+    for (int d = 0; d < 3; ++d) p.bounds.bound[d] = prm->bound[d];
+    p.bounds.particle_radius = prm->particle_radius;
+    p.bounds.boundary_epsilon = prm->boundary_epsilon;
+    return FS_OK;
+}
 
-    // predict + bounds fused into the build's hash (solver_pbf.py:702-737)
+// The stream holding the predicted positions before iteration `it` (it =
+// max_iterations: after the loop).  The non-pressure pass writes stream 1;
+// each delta pass writes the other one; a per-iteration rebuild gathers into
+// stream 0.
+static int pbf_cur(const fs_pbf_params* prm, int it)
+{
+    if (prm->rebuild_grid_each_iteration) return it == 0 ? 1 : 0;
+    return (1 + it) & 1;
+}
+
+// predict_velocity_with_gravity + predict_positions + apply_position_bounds
+// fused into the build's hash, then the build (solver_pbf.py:702-737)
+static int32_t pbf_predict(const PbfPlan& p)
+{
+    const fs_model* m = p.m;
     PreOp pre;
     pre.mode = kPrePredict;
-    pre.x_in = io->x_in;
-    pre.v_in = io->v_in;
-    pre.x_out = w->x_pred;
-    pre.v_out = w->v_pred;
+    pre.x_in = p.io->x_in;
+    pre.v_in = p.io->v_in;
+    pre.x_out = p.w->x_pred;
+    pre.v_out = p.w->v_pred;
     pre.flags = m->flags;
     pre.mask = m->mask;
     pre.world = m->world;
     pre.gravity = m->gravity;
-    pre.dt = dt;
-    for (int d = 0; d < 3; ++d) pre.bound[d] = prm->bound[d];
-    pre.particle_radius = prm->particle_radius;
-    pre.boundary_epsilon = prm->boundary_epsilon;
-    rc = build_grid(g, n, h, pre, gather_for(m, w, w->x_pred, w->v_pred, vel), s);
-    if (rc) return rc;
+    pre.dt = p.dt;
+    for (int d = 0; d < 3; ++d) pre.bound[d] = p.prm->bound[d];
+    pre.particle_radius = p.prm->particle_radius;
+    pre.boundary_epsilon = p.prm->boundary_epsilon;
+    return build_grid(p.g, p.n, p.h, pre, gather_for(m, p.w, p.w->x_pred, p.w->v_pred, p.vel), p.s);
+}
 
-    // Positions change after the non-pressure pass and after every
-    // delta-position pass: the pass that follows a change scans (and
-    // records), the one after it, at the same positions, replays.
+// Positions change after the non-pressure pass and after every delta pass:
+// the pass that follows a change scans (and records), the one after it, at
+// the same positions, replays.
+//
+// compute_density (solver_pbf.py:738-751), or with `final_pass` the density
+// at the final positions + update_velocity_from_positions (+ apply_bounds
+// when there is no XSPH pass)  (:878-917, :939-954)
+static int32_t pbf_density(const PbfPlan& p, bool final_pass)
+{
+    const int cur = final_pass ? pbf_cur(p.prm, p.prm->max_iterations) : 0;
     PbfDensityArgs da;
-    da.c = make_common(g, w, bufs[cur], h, kListBuild);
-    da.wvol = w->wvol;
-    da.rho = io->rho;
-    da.sca = sca;
-    da.final_pass = 0;
-    da.finalize = 0;
-    da.x_in = io->x_in;
-    da.v_in = io->v_in;
-    da.dt = dt;
-    da.damping = prm->boundary_damping;
-    da.bounds = bounds;
-    da.vel = vel;
-    da.x_out = io->x_out;
-    da.v_out = io->v_out;
-    FS_LAUNCH((k_pbf_density<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, da);   // :738-751
+    da.c = make_common(p.g, p.w, p.bufs[cur], p.h, kListBuild);
+    da.wvol = p.w->wvol;
+    da.rho = p.io->rho;
+    da.sca = p.sca;
+    da.final_pass = final_pass ? 1 : 0;
+    da.finalize = (final_pass && !(p.prm->xsph_c > 0.0f)) ? 1 : 0;
+    da.x_in = p.io->x_in;
+    da.v_in = p.io->v_in;
+    da.dt = p.dt;
+    da.damping = p.prm->boundary_damping;
+    da.bounds = p.bounds;
+    da.vel = p.vel;
+    da.x_out = p.io->x_out;
+    da.v_out = p.io->v_out;
+    FS_LAUNCH((k_pbf_density<kScanBlock, kScanCap>), dim3(blocks_for(p.n, kScanBlock)), dim3(kScanBlock), 0, p.s, da);
+    return FS_OK;
+}
 
+// acceleration_non_pressure + apply_non_pressure_prediction +
+// apply_position_bounds  (solver_pbf.py:752-796)
+static int32_t pbf_nonpressure(const PbfPlan& p)
+{
     PbfNonPressureArgs na;
-    na.c = make_common(g, w, bufs[cur], h, kListUse);
-    na.vel = vel;
-    na.surface_tension = m->surface_tension;
-    na.viscosity = m->viscosity;
-    na.particle_radius = prm->particle_radius;
-    na.dt = dt;
-    na.bounds = bounds;
-    na.accel = io->accel;
-    na.x_pred = w->x_pred;
-    na.v_pred = w->v_pred;
-    na.pos_next = bufs[cur ^ 1];
-    LAUNCH_PASS(k_pbf_nonpressure<kReplayCap>, nb, s, na);   // :752-796
-    cur ^= 1;
+    na.c = make_common(p.g, p.w, p.bufs[0], p.h, kListUse);
+    na.vel = p.vel;
+    na.surface_tension = p.m->surface_tension;
+    na.viscosity = p.m->viscosity;
+    na.particle_radius = p.prm->particle_radius;
+    na.dt = p.dt;
+    na.bounds = p.bounds;
+    na.accel = p.io->accel;
+    na.x_pred = p.w->x_pred;
+    na.v_pred = p.w->v_pred;
+    na.pos_next = p.bufs[1];
+    LAUNCH_PASS(k_pbf_nonpressure<kReplayCap>, p.nb, p.s, na);
+    return FS_OK;
+}
 
-    for (int it = 0; it < prm->max_iterations; ++it) {   // :800-858
-        PbfLambdaArgs la;
-        la.c = make_common(g, w, bufs[cur], h, kListBuild);
-        la.rest_density = m->rest_density;
-        la.lambda_epsilon = prm->lambda_epsilon;
-        la.min_neighbors = prm->min_neighbors_for_lambda;
-        la.pbf_lambda = io->pbf_lambda;
-        la.rho = io->rho;
-        la.neighbor_count = io->neighbor_count;
-        la.sca = sca;
-        FS_LAUNCH((k_pbf_lambda<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, la);
+// compute_pbf_lambda of iteration `it`  (solver_pbf.py:801-817)
+static int32_t pbf_lambda(const PbfPlan& p, int it)
+{
+    PbfLambdaArgs la;
+    la.c = make_common(p.g, p.w, p.bufs[pbf_cur(p.prm, it)], p.h, kListBuild);
+    la.rest_density = p.m->rest_density;
+    la.lambda_epsilon = p.prm->lambda_epsilon;
+    la.min_neighbors = p.prm->min_neighbors_for_lambda;
+    la.pbf_lambda = p.io->pbf_lambda;
+    la.rho = p.io->rho;
+    la.neighbor_count = p.io->neighbor_count;
+    la.sca = p.sca;
+    FS_LAUNCH((k_pbf_lambda<kScanBlock, kScanCap>), dim3(blocks_for(p.n, kScanBlock)), dim3(kScanBlock), 0, p.s, la);
+    return FS_OK;
+}
 
-        PbfDeltaArgs pa;
-        pa.c = make_common(g, w, bufs[cur], h, kListUse);
-        pa.sca = sca;
-        pa.rest_density = m->rest_density;
-        pa.scorr_k = prm->scorr_k;
-        pa.scorr_n = prm->scorr_n;
-        pa.scorr_w_delta_q = prm->scorr_w_delta_q;
-        pa.max_delta_position = prm->max_delta_position;
-        pa.apply_bounds = (it == prm->max_iterations - 1 || prm->rebuild_grid_each_iteration) ? 1 : 0;
-        pa.bounds = bounds;
-        pa.delta_p = io->delta_p;
-        pa.x_pred = w->x_pred;
-        pa.pos_next = bufs[cur ^ 1];
-        if (prm->scorr_k != 0.0f) LAUNCH_PASS((k_pbf_delta<kReplayCap, true>), nb, s, pa);
-        else LAUNCH_PASS((k_pbf_delta<kReplayCap, false>), nb, s, pa);
-        cur ^= 1;
-        if (prm->rebuild_grid_each_iteration) {
-            PreOp rp;
-            rp.x_in = w->x_pred;
-            rc = build_grid(g, n, h, rp, gather_for(m, w, w->x_pred, nullptr, nullptr), s);
-            if (rc) return rc;
-            cur = 0;   // the gather writes w->pos
-        }
-    }
-
-    const bool xsph = prm->xsph_c > 0.0f;
-    da.c = make_common(g, w, bufs[cur], h, kListBuild);
-    da.final_pass = 1;
-    da.finalize = xsph ? 0 : 1;
-    FS_LAUNCH((k_pbf_density<kScanBlock, kScanCap>), dim3(blocks_for(n, kScanBlock)), dim3(kScanBlock), 0, s, da);   // :878-917 (+ :939-954 without XSPH)
-    if (xsph) {
-        PbfXsphArgs xa;
-        xa.c = make_common(g, w, bufs[cur], h, kListUse);
-        xa.vel = vel;
-        xa.xsph_c = prm->xsph_c;
-        xa.damping = prm->boundary_damping;
-        xa.bounds = bounds;
-        xa.x_out = io->x_out;
-        xa.v_out = io->v_out;
-        LAUNCH_PASS(k_pbf_xsph<kReplayCap>, nb, s, xa);   // :919-954
+// compute_delta_position + apply_delta_position (+ apply_position_bounds and
+// the rebuild when rebuilding each iteration, or after the last iteration)
+// of iteration `it`  (solver_pbf.py:818-875)
+static int32_t pbf_delta(const PbfPlan& p, int it)
+{
+    const fs_pbf_params* prm = p.prm;
+    const int cur = pbf_cur(prm, it);
+    PbfDeltaArgs pa;
+    pa.c = make_common(p.g, p.w, p.bufs[cur], p.h, kListUse);
+    pa.sca = p.sca;
+    pa.rest_density = p.m->rest_density;
+    pa.scorr_k = prm->scorr_k;
+    pa.scorr_n = prm->scorr_n;
+    pa.scorr_w_delta_q = prm->scorr_w_delta_q;
+    pa.max_delta_position = prm->max_delta_position;
+    pa.apply_bounds = (it == prm->max_iterations - 1 || prm->rebuild_grid_each_iteration) ? 1 : 0;
+    pa.bounds = p.bounds;
+    pa.delta_p = p.io->delta_p;
+    pa.x_pred = p.w->x_pred;
+    pa.pos_next = p.bufs[cur ^ 1];
+    if (prm->scorr_k != 0.0f) LAUNCH_PASS((k_pbf_delta<kReplayCap, true>), p.nb, p.s, pa);
+    else LAUNCH_PASS((k_pbf_delta<kReplayCap, false>), p.nb, p.s, pa);
+    if (prm->rebuild_grid_each_iteration) {
+        PreOp rp;
+        rp.x_in = p.w->x_pred;
+        return build_grid(p.g, p.n, p.h, rp, gather_for(p.m, p.w, p.w->x_pred, nullptr, nullptr), p.s);
     }
     return FS_OK;
+}
+
+// apply_xsph_viscosity + apply_bounds  (solver_pbf.py:919-954); without XSPH
+// (xsph_c <= 0) the final density pass has finished the substep and this
+// pass does nothing
+static int32_t pbf_xsph_finalize(const PbfPlan& p)
+{
+    if (!(p.prm->xsph_c > 0.0f)) return FS_OK;
+    PbfXsphArgs xa;
+    xa.c = make_common(p.g, p.w, p.bufs[pbf_cur(p.prm, p.prm->max_iterations)], p.h, kListUse);
+    xa.vel = p.vel;
+    xa.xsph_c = p.prm->xsph_c;
+    xa.damping = p.prm->boundary_damping;
+    xa.bounds = p.bounds;
+    xa.x_out = p.io->x_out;
+    xa.v_out = p.io->v_out;
+    LAUNCH_PASS(k_pbf_xsph<kReplayCap>, p.nb, p.s, xa);
+    return FS_OK;
+}
+
+int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,
+                    fs_stream_t stream)
+{
+    PbfPlan p;
+    int32_t rc = pbf_plan(g, w, m, prm, io, stream, "fs_pbf_step", p);
+    if (rc) return rc;
+    if (p.n == 0) return FS_OK;
+#define PBF_TRY(call)      \
+    do {                   \
+        rc = (call);       \
+        if (rc) return rc; \
+    } while (0)
+    PBF_TRY(pbf_predict(p));
+    PBF_TRY(pbf_density(p, false));
+    PBF_TRY(pbf_nonpressure(p));
+    for (int it = 0; it < prm->max_iterations; ++it) {   // solver_pbf.py:800-858
+        PBF_TRY(pbf_lambda(p, it));
+        PBF_TRY(pbf_delta(p, it));
+    }
+    PBF_TRY(pbf_density(p, true));
+    return pbf_xsph_finalize(p);
+#undef PBF_TRY
+}
+
+// the passes of fs_pbf_step, one call each
+static int32_t pbf_pass(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,
+                        fs_stream_t stream, const char* what, int which, int iteration)
+{
+    PbfPlan p;
+    int32_t rc = pbf_plan(g, w, m, prm, io, stream, what, p);
+    if (rc) return rc;
+    if (p.n == 0) return FS_OK;
+    if (which != 0 && g->num_points != m->n) {
+        set_error("%s: the grid does not hold this model's particles (run fs_pbf_predict first)", what);
+        return FS_ERR_ARG;
+    }
+    if ((which == 3 || which == 4) && (iteration < 0 || iteration >= prm->max_iterations)) {
+        set_error("%s: iteration %d outside [0, max_iterations = %d)", what, iteration, prm->max_iterations);
+        return FS_ERR_ARG;
+    }
+    switch (which) {
+    case 0: return pbf_predict(p);
+    case 1: return pbf_density(p, false);
+    case 2: return pbf_nonpressure(p);
+    case 3: return pbf_lambda(p, iteration);
+    case 4: return pbf_delta(p, iteration);
+    case 5: return pbf_density(p, true);
+    default: return pbf_xsph_finalize(p);
+    }
+}
+
+int32_t fs_dfsph_density_factor(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm,
+                                const fs_dfsph_io* io, int32_t after_integrate, fs_stream_t stream)
+{
+    return df_pass(g, w, m, prm, io, stream, "fs_dfsph_density_factor", 0, after_integrate ? 1 : 0, 0);
+}
+int32_t fs_dfsph_nonpressure_predict(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm,
+                                     const fs_dfsph_io* io, fs_stream_t stream)
+{
+    return df_pass(g, w, m, prm, io, stream, "fs_dfsph_nonpressure_predict", 1, 0, 0);
+}
+int32_t fs_dfsph_rate_kappa(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm,
+                            const fs_dfsph_io* io, int32_t loop, fs_stream_t stream)
+{
+    return df_pass(g, w, m, prm, io, stream, "fs_dfsph_rate_kappa", 2, loop, 0);
+}
+int32_t fs_dfsph_residual(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm,
+                          const fs_dfsph_io* io, int32_t loop, int32_t iteration, fs_stream_t stream)
+{
+    return df_pass(g, w, m, prm, io, stream, "fs_dfsph_residual", 3, loop, iteration);
+}
+int32_t fs_dfsph_correction(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm,
+                            const fs_dfsph_io* io, int32_t loop, int32_t warm_start, fs_stream_t stream)
+{
+    return df_pass(g, w, m, prm, io, stream, "fs_dfsph_correction", 4, loop, warm_start ? -1 : 0);
+}
+int32_t fs_dfsph_integrate(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm,
+                           const fs_dfsph_io* io, fs_stream_t stream)
+{
+    return df_pass(g, w, m, prm, io, stream, "fs_dfsph_integrate", 5, 0, 0);
+}
+int32_t fs_dfsph_finalize(fs_grid* g, fs_work* w, const fs_model* m, const fs_dfsph_params* prm,
+                          const fs_dfsph_io* io, fs_stream_t stream)
+{
+    return df_pass(g, w, m, prm, io, stream, "fs_dfsph_finalize", 6, 0, 0);
+}
+
+int32_t fs_pbf_predict(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,
+                       fs_stream_t stream)
+{
+    return pbf_pass(g, w, m, prm, io, stream, "fs_pbf_predict", 0, 0);
+}
+int32_t fs_pbf_density(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,
+                       fs_stream_t stream)
+{
+    return pbf_pass(g, w, m, prm, io, stream, "fs_pbf_density", 1, 0);
+}
+int32_t fs_pbf_nonpressure(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm,
+                           const fs_pbf_io* io, fs_stream_t stream)
+{
+    return pbf_pass(g, w, m, prm, io, stream, "fs_pbf_nonpressure", 2, 0);
+}
+int32_t fs_pbf_lambda(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,
+                      int32_t iteration, fs_stream_t stream)
+{
+    return pbf_pass(g, w, m, prm, io, stream, "fs_pbf_lambda", 3, iteration);
+}
+int32_t fs_pbf_delta(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,
+                     int32_t iteration, fs_stream_t stream)
+{
+    return pbf_pass(g, w, m, prm, io, stream, "fs_pbf_delta", 4, iteration);
+}
+int32_t fs_pbf_final_density(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm,
+                             const fs_pbf_io* io, fs_stream_t stream)
+{
+    return pbf_pass(g, w, m, prm, io, stream, "fs_pbf_final_density", 5, 0);
+}
+int32_t fs_pbf_xsph_finalize(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm,
+                             const fs_pbf_io* io, fs_stream_t stream)
+{
+    return pbf_pass(g, w, m, prm, io, stream, "fs_pbf_xsph_finalize", 6, 0);
 }
 
 }  // extern "C"

@@ -224,6 +224,14 @@ typedef struct fs_wcsph_io {
  * (SolverWCSPH.step, solver_wcsph.py:431-535). */
 int32_t fs_wcsph_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_wcsph_params* params,
                       const fs_wcsph_io* io, fs_stream_t stream);
+/* fs_wcsph_step is exactly these two passes, in order, on one stream:
+ * build + compute_density_pressure (solver_wcsph.py:456-474; records the
+ * neighbour lists), then acceleration_wcsph + integrate_velocity +
+ * integrate_positions + apply_bounds (:475-535; replays them). */
+int32_t fs_wcsph_density_eos(fs_grid* grid, fs_work* work, const fs_model* model, const fs_wcsph_params* params,
+                             const fs_wcsph_io* io, fs_stream_t stream);
+int32_t fs_wcsph_force_integrate(fs_grid* grid, fs_work* work, const fs_model* model,
+                                 const fs_wcsph_params* params, const fs_wcsph_io* io, fs_stream_t stream);
 
 /* WCSPH adjoint.  No counterpart in the reference, whose gradients would come
  * from Warp's autodiff tape; this is the reverse-mode derivative of
@@ -302,6 +310,35 @@ typedef struct fs_dfsph_io {
  * float64; the remaining passes of a finished loop return at once. */
 int32_t fs_dfsph_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_dfsph_params* params,
                       const fs_dfsph_io* io, fs_stream_t stream);
+/* The passes of fs_dfsph_step, one call each; the substep is exactly:
+ *   fs_dfsph_density_factor(after_integrate = 0)   build on x_in + compute_density + compute_factor_dfsph (:624-653)
+ *   fs_dfsph_nonpressure_predict                   acceleration_non_pressure + predict_velocity (:654-688)
+ *   [fs_dfsph_correction(loop 0, warm_start = 1)]  when warm starting (:689-707)
+ *   max_rho_iterations x:                          (:711-767)
+ *     fs_dfsph_rate_kappa(loop 0)                  compute_density_change_rate + predict_density_star + compute_kappa_density
+ *     fs_dfsph_residual(loop 0, iteration)         _density_residual and the break rule, decided on the device
+ *     fs_dfsph_correction(loop 0, warm_start = 0)  apply_constant_density_correction
+ *   fs_dfsph_integrate                             integrate_positions + build on x_out (:769-783)
+ *   fs_dfsph_density_factor(after_integrate = 1)   compute_density + compute_factor_dfsph (:785-812)
+ *   [fs_dfsph_correction(loop 1, warm_start = 1)]  when warm starting (:814-832)
+ *   max_vel_iterations x: rate_kappa, residual, correction with loop 1 (:834-883)
+ *   fs_dfsph_finalize                              apply_bounds and the outputs (:886-898)
+ * Once a loop's residual has stopped it, that loop's later rate and
+ * correction calls return at once (on the device). */
+int32_t fs_dfsph_density_factor(fs_grid* grid, fs_work* work, const fs_model* model, const fs_dfsph_params* params,
+                                const fs_dfsph_io* io, int32_t after_integrate, fs_stream_t stream);
+int32_t fs_dfsph_nonpressure_predict(fs_grid* grid, fs_work* work, const fs_model* model,
+                                     const fs_dfsph_params* params, const fs_dfsph_io* io, fs_stream_t stream);
+int32_t fs_dfsph_rate_kappa(fs_grid* grid, fs_work* work, const fs_model* model, const fs_dfsph_params* params,
+                            const fs_dfsph_io* io, int32_t loop, fs_stream_t stream);
+int32_t fs_dfsph_residual(fs_grid* grid, fs_work* work, const fs_model* model, const fs_dfsph_params* params,
+                          const fs_dfsph_io* io, int32_t loop, int32_t iteration, fs_stream_t stream);
+int32_t fs_dfsph_correction(fs_grid* grid, fs_work* work, const fs_model* model, const fs_dfsph_params* params,
+                            const fs_dfsph_io* io, int32_t loop, int32_t warm_start, fs_stream_t stream);
+int32_t fs_dfsph_integrate(fs_grid* grid, fs_work* work, const fs_model* model, const fs_dfsph_params* params,
+                           const fs_dfsph_io* io, fs_stream_t stream);
+int32_t fs_dfsph_finalize(fs_grid* grid, fs_work* work, const fs_model* model, const fs_dfsph_params* params,
+                          const fs_dfsph_io* io, fs_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* PBF                                                                      */
@@ -340,6 +377,31 @@ typedef struct fs_pbf_io {
 /* One PBF substep (SolverPBF.step, solver_pbf.py:676-954). */
 int32_t fs_pbf_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_pbf_params* params,
                     const fs_pbf_io* io, fs_stream_t stream);
+/* The passes of fs_pbf_step, one call each; the substep is exactly:
+ *   fs_pbf_predict          predict_velocity_with_gravity + predict_positions + apply_position_bounds + build (:702-737)
+ *   fs_pbf_density          compute_density (:738-751)
+ *   fs_pbf_nonpressure      acceleration_non_pressure + apply_non_pressure_prediction + bounds (:752-796)
+ *   max_iterations x:       (:800-858)
+ *     fs_pbf_lambda(i)      compute_pbf_lambda
+ *     fs_pbf_delta(i)       compute_delta_position + apply_delta_position (+ bounds and rebuild as configured)
+ *   fs_pbf_final_density    compute_density + update_velocity_from_positions (+ apply_bounds without XSPH) (:878-917)
+ *   fs_pbf_xsph_finalize    apply_xsph_viscosity + apply_bounds; nothing when xsph_c <= 0 (:919-954)
+ * Predicted positions alternate between the workspace's two sorted position
+ * streams; which one a pass reads follows from the iteration. */
+int32_t fs_pbf_predict(fs_grid* grid, fs_work* work, const fs_model* model, const fs_pbf_params* params,
+                       const fs_pbf_io* io, fs_stream_t stream);
+int32_t fs_pbf_density(fs_grid* grid, fs_work* work, const fs_model* model, const fs_pbf_params* params,
+                       const fs_pbf_io* io, fs_stream_t stream);
+int32_t fs_pbf_nonpressure(fs_grid* grid, fs_work* work, const fs_model* model, const fs_pbf_params* params,
+                           const fs_pbf_io* io, fs_stream_t stream);
+int32_t fs_pbf_lambda(fs_grid* grid, fs_work* work, const fs_model* model, const fs_pbf_params* params,
+                      const fs_pbf_io* io, int32_t iteration, fs_stream_t stream);
+int32_t fs_pbf_delta(fs_grid* grid, fs_work* work, const fs_model* model, const fs_pbf_params* params,
+                     const fs_pbf_io* io, int32_t iteration, fs_stream_t stream);
+int32_t fs_pbf_final_density(fs_grid* grid, fs_work* work, const fs_model* model, const fs_pbf_params* params,
+                             const fs_pbf_io* io, fs_stream_t stream);
+int32_t fs_pbf_xsph_finalize(fs_grid* grid, fs_work* work, const fs_model* model, const fs_pbf_params* params,
+                             const fs_pbf_io* io, fs_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* APIC particle separation                                                 */
