@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restric
 
 __global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __restrict__ cell_of,
                                                         const int* __restrict__ rank, const int* __restrict__ start,
-                                                        int* __restrict__ scatter_ids, int* __restrict__ count,
+                                                        int2* __restrict__ scatter, int* __restrict__ count,
                                                         uint64_t* scan_desc_clear, int ntiles)
 {
     const int i = blockIdx.x * kElemBlock + threadIdx.x;
@@ -349,22 +349,24 @@ __global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __rest
     if (i >= n) return;
     int c = cell_of[i];
     int slot = start[c] + rank[i];
-    if ((unsigned)slot < (unsigned)n) scatter_ids[slot] = i;
+    // (index, bucket): the fix-up reads the bucket with the index instead of
+    // a dependent cell_of[i] load
+    if ((unsigned)slot < (unsigned)n) scatter[slot] = make_int2(i, c);
     count[c] = 0;
 }
 
-__global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int* __restrict__ scatter_ids,
-                                                           const int* __restrict__ cell_of,
+__global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __restrict__ scatter,
                                                            const int* __restrict__ start, int* __restrict__ point_ids,
                                                            int* __restrict__ point_cells, GatherOp ga)
 {
     const int k = blockIdx.x * kElemBlock + threadIdx.x;
     if (k >= n) return;
-    const int i = scatter_ids[k];
-    const int c = cell_of[i];
+    const int2 e = scatter[k];
+    const int i = e.x;
+    const int c = e.y;
     const int a = min(max(start[c], 0), n), b = min(max(start[c + 1], a), n);
     int r = 0;
-    for (int m = a; m < b; ++m) r += (scatter_ids[m] < i) ? 1 : 0;
+    for (int m = a; m < b; ++m) r += (scatter[m].x < i) ? 1 : 0;
     const int f = a + r;
     point_ids[f] = i;
     point_cells[f] = c;
@@ -477,11 +479,12 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
     FS_LAUNCH(k_bucket_scan, dim3(ntiles), dim3(kScanBlock), 0, s, g->cell_count, g->cell_start, m, ntiles,
               g->scan_state);
     const int nbs = nb > blocks_for(ntiles, kElemBlock) ? nb : blocks_for(ntiles, kElemBlock);
-    FS_LAUNCH(k_scatter, dim3(nbs), dim3(kElemBlock), 0, s, n, g->cell_of, g->rank, g->cell_start, g->scatter_ids,
+    FS_LAUNCH(k_scatter, dim3(nbs), dim3(kElemBlock), 0, s, n, g->cell_of, g->rank, g->cell_start,
+              reinterpret_cast<int2*>(g->scatter_ids),
               g->cell_count, g->scan_state, ntiles);
     if (nb == 0) return FS_OK;
-    FS_LAUNCH(k_fix_gather, dim3(nb), dim3(kElemBlock), 0, s, n, g->scatter_ids, g->cell_of, g->cell_start,
-              g->point_ids, g->point_cells, ga);
+    FS_LAUNCH(k_fix_gather, dim3(nb), dim3(kElemBlock), 0, s, n, reinterpret_cast<const int2*>(g->scatter_ids),
+              g->cell_start, g->point_ids, g->point_cells, ga);
     return FS_OK;
 }
 
@@ -531,7 +534,8 @@ size_t fs_grid_workspace_bytes(int32_t capacity, int32_t dim_x, int32_t dim_y, i
     size_t cap = (size_t)(capacity > 0 ? capacity : 1);
     size_t ntiles = (m + kScanTile - 1) / kScanTile;
     size_t b = 0;
-    b += align_up(cap * 4) * 5;                 // point_ids, point_cells, cell_of, rank, scatter_ids
+    b += align_up(cap * 4) * 4;                 // point_ids, point_cells, cell_of, rank
+    b += align_up(cap * 8);                     // scatter_ids: (index, bucket) pairs
     b += align_up((m + 1) * 4);                 // cell_start
     b += align_up(m * 4);                       // cell_count
     b += align_up(ntiles * 8);                  // scan_state
@@ -579,7 +583,7 @@ int32_t fs_grid_init(fs_grid* g, void* workspace, size_t bytes, int32_t capacity
     g->point_cells = (int32_t*)take(cap * 4);
     g->cell_of = (int32_t*)take(cap * 4);
     g->rank = (int32_t*)take(cap * 4);
-    g->scatter_ids = (int32_t*)take(cap * 4);
+    g->scatter_ids = (int32_t*)take(cap * 8);
     g->cell_start = (int32_t*)take((m + 1) * 4);
     g->cell_count = (int32_t*)take(m * 4);
     g->scan_state = (uint64_t*)take(ntiles * 8);

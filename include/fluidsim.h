@@ -101,7 +101,7 @@ typedef struct fs_grid {
     int32_t* cell_start;    /* [dims+1] exclusive prefix over buckets           */
     int32_t* cell_count;    /* [dims] per-bucket counters, zero between builds  */
     int32_t* rank;          /* [capacity] scratch                              */
-    int32_t* scatter_ids;   /* [capacity] scratch                              */
+    int32_t* scatter_ids;   /* [2*capacity] scratch: (index, bucket) pairs     */
     uint64_t* scan_state;   /* look-back descriptors of the bucket scan         */
     uint32_t* sync;         /* [8] self-resetting counters                      */
 } fs_grid;
