@@ -21,7 +21,10 @@ whole WCSPH frames with the host CFL driver beside the device-resident one,
 and the APIC particle-separation pass over the jittered dam-break lattice.
 ``roofline.traffic`` is the HBM bytes per launch of the dominant kernel, read
 from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary for the launch
-variant the autotuner selected.
+variant the autotuner selected.  ``valu_issue`` puts the same launch against
+the roof that binds it: SQ_INSTS_VALU per launch (committed SQ summary) over
+the live launch duration, against one wave64 VALU instruction per CU per
+cycle.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--solvers wcsph,dfsph,pbf]
        [--no-tune] [--no-extras] [--no-cpu-baseline]
@@ -78,6 +81,28 @@ def pmc_traffic(kernel: str, block: int, depth: int):
                 if key in line and line.startswith("|"):
                     return float(line.strip().strip("|").split("|")[-1])
     except OSError:
+        pass
+    return None
+
+
+PMC_SQ = os.path.join(ROOT, "profiles", "wcsph_dam_break_401856_pmc_sq.md")
+# wave64 FP32 VALU issue of the whole chip: one instruction per CU per cycle
+# (4 SIMDs of 16 lanes, 4 cycles each), 256 CUs at 2.4 GHz
+VALU_ISSUE_PEAK = 256 * 2.4e9
+
+
+def pmc_valu_instructions(kernel: str, block: int, depth: int):
+    """SQ_INSTS_VALU per launch of one launch variant (waves x VALU/wave), from
+    the committed rocprofv3 SQ summary; None when the variant was not
+    measured."""
+    key = f"{kernel}<{block}, {depth}, false>"
+    try:
+        with open(PMC_SQ) as f:
+            for line in f:
+                if key in line and line.startswith("|"):
+                    cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                    return float(cells[2].replace(",", "")) * float(cells[3].replace(",", ""))
+    except (OSError, ValueError, IndexError):
         pass
     return None
 
@@ -699,6 +724,15 @@ def main():
                 "bytes_per_launch": PASS_BYTES[dom] * scene.count,
                 "traffic_source": "profiles/wcsph_dam_break_401856_pmc_traffic.md (HBM bytes per launch of the "
                                   "selected launch variant: FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"}
+    valu = None
+    if tkey in tuned:
+        instr = pmc_valu_instructions(dom, tuned[tkey]["block"], tuned[tkey]["list_depth"])
+        if instr:
+            rate = instr / avg_s
+            valu = {"bound": "valu_issue", "kernel": dom, "instructions_per_launch": instr, "achieved": rate,
+                    "peak": VALU_ISSUE_PEAK, "unit": "wave64 VALU instructions/s", "frac": rate / VALU_ISSUE_PEAK,
+                    "source": "profiles/wcsph_dam_break_401856_pmc_sq.md (SQ_INSTS_VALU per launch of the selected "
+                              "variant) over the live HIP-event launch duration"}
     extras = {}
     if not args.no_extras and args.particles == 400000:
         print("# batched form: 64 scenes", file=sys.stderr, flush=True)
@@ -733,6 +767,7 @@ def main():
                    "particles": scene.count, "solver": head, "dt": headline["dt"]},
         "hbm_frac_substep": headline["hbm_frac"],
         "roofline": roofline,
+        "valu_issue": valu,
         "cpu_baseline": cpu,
         "cpu_baseline_detail": cpu_detail,
         "solvers": results,
