@@ -496,39 +496,38 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_force(WadjForceArgs a)
     REPLAY_NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, phased(
         [&](int j) { return AdjNbr{c.sflags[j], c.pos[j], a.abar[j], a.vel[j], a.sca[j].z}; },
         [&](int j, const AdjNbr& nb) {
-        if (j == s || !(nb.flags & kActive)) return;
-        const float4 xj4 = nb.x;
-        const v3 r = vsub(xk, xyz(xj4));
-        const float d = vlen(r);
-        if (!(d / h <= 1.0f)) return;
-        const v3 G = cubic_grad_r(r, d, h);
-        const float4 aj4 = nb.abar;
-        const float4 vj4 = nb.v;
-        const float4 sj = make_float4(0.0f, 0.0f, nb.P, 0.0f);
-        const v3 aj = xyz(aj4);
-        const float m_j = xj4.w;
-        const float P_j = sj.z;
-        const float w_j = vj4.w;
-        const float c_j = aj4.w;
-        const float den = d * d + eps2;
-        const v3 dv = vsub(vk, xyz(vj4));
-        const float sv = vdot(dv, r) / den;
-        const float Gak = vdot(G, ak);
-        const float Gaj = vdot(G, aj);
-        const v3 Hak = cubic_hess_mul(r, d, h, ak);
-        const v3 Haj = cubic_hess_mul(r, d, h, aj);
-        const v3 ds = vdivs(vsub(dv, smul(2.0f * sv, r)), den);
-        xb = vadd(xb, smul(-m_j * (P_k + P_j), Hak));
-        xb = vadd(xb, smul(m_k * (P_j + P_k), Haj));
-        Pb += -m_j * Gak + m_k * Gaj;
-        const float ck_wj = c_k * w_j;
-        const float cj_wk = c_j * w_k;
-        xb = vadd(xb, smul(ck_wj, vadd(smul(Gak, ds), smul(sv, Hak))));
-        xb = vsub(xb, smul(cj_wk, vadd(smul(Gaj, ds), smul(sv, Haj))));
-        vb = vadd(vb, smul((ck_wj * Gak - cj_wk * Gaj) / den, r));
-        mub += 10.0f * w_j * sv * Gak;
-        wb += -c_j * sv * Gaj;
-    }));
+            if (j == s || !(nb.flags & kActive)) return;
+            const float4 xj4 = nb.x;
+            const v3 r = vsub(xk, xyz(xj4));
+            const float d = vlen(r);
+            if (!(d / h <= 1.0f)) return;
+            const v3 G = cubic_grad_r(r, d, h);
+            const float4 aj4 = nb.abar;
+            const float4 vj4 = nb.v;
+            const v3 aj = xyz(aj4);
+            const float m_j = xj4.w;
+            const float P_j = nb.P;
+            const float w_j = vj4.w;
+            const float c_j = aj4.w;
+            const float den = d * d + eps2;
+            const v3 dv = vsub(vk, xyz(vj4));
+            const float sv = vdot(dv, r) / den;
+            const float Gak = vdot(G, ak);
+            const float Gaj = vdot(G, aj);
+            const v3 Hak = cubic_hess_mul(r, d, h, ak);
+            const v3 Haj = cubic_hess_mul(r, d, h, aj);
+            const v3 ds = vdivs(vsub(dv, smul(2.0f * sv, r)), den);
+            xb = vadd(xb, smul(-m_j * (P_k + P_j), Hak));
+            xb = vadd(xb, smul(m_k * (P_j + P_k), Haj));
+            Pb += -m_j * Gak + m_k * Gaj;
+            const float ck_wj = c_k * w_j;
+            const float cj_wk = c_j * w_k;
+            xb = vadd(xb, smul(ck_wj, vadd(smul(Gak, ds), smul(sv, Hak))));
+            xb = vsub(xb, smul(cj_wk, vadd(smul(Gaj, ds), smul(sv, Haj))));
+            vb = vadd(vb, smul((ck_wj * Gak - cj_wk * Gaj) / den, r));
+            mub += 10.0f * w_j * sv * Gak;
+            wb += -c_j * sv * Gaj;
+        }));
     float rb = 0.0f;
     const float pb = Pb / fmax_(rho_k * rho_k, 1.0e-6f);
     if (rho_k * rho_k > 1.0e-6f) rb += Pb * (-2.0f * p_k / (rho_k * rho_k * rho_k));
@@ -1103,17 +1102,17 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_lambda(PbfLambdaArgs a)
     SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
         [&](int k) { return NbrX{c.sflags[k], c.pos[k]}; },
         [&](int k, const NbrX& nb) {
-        if (!(nb.flags & kActive)) return;
-        const float4 xj = nb.x;
-        v3 xij = vsub(xi, xyz(xj));
-        float r = vlen(xij);
-        if (k != s && r < h) count += 1;
-        float mass_j = xj.w;
-        rho_i += mass_j * poly6_w(r, h);
-        v3 grad_j = smul(-(mass_j / rho0_i), spiky_grad_r(xij, r, h));
-        sum_grad_sq += vdot(grad_j, grad_j);
-        grad_i = vsub(grad_i, grad_j);
-    }));
+            if (!(nb.flags & kActive)) return;
+            const float4 xj = nb.x;
+            v3 xij = vsub(xi, xyz(xj));
+            float r = vlen(xij);
+            if (k != s && r < h) count += 1;
+            float mass_j = xj.w;
+            rho_i += mass_j * poly6_w(r, h);
+            v3 grad_j = smul(-(mass_j / rho0_i), spiky_grad_r(xij, r, h));
+            sum_grad_sq += vdot(grad_j, grad_j);
+            grad_i = vsub(grad_i, grad_j);
+        }));
     sum_grad_sq += vdot(grad_i, grad_i);
     const float c_i = fmax_(rho_i / rho0_i - 1.0f, 0.0f);
     a.rho[i] = rho_i;
