@@ -243,11 +243,21 @@ __device__ __forceinline__ int bucket_of(const GridView& g, float x, float y, fl
 // Iterator over the sorted-slot ranges visited by hash_grid_query(p, r):
 // rows (y,z) in Warp order; within a row the x run is one contiguous range
 // of sorted slots, or two when the run wraps past the last bucket.
+// The hashed y/z of successive rows are stepped, not recomputed: moving from
+// virtual coordinate c-1 to c, hash_axis adds one and wraps at the dimension,
+// except that coordinates at or below -kOrigin all hash to 0.  The row bases
+// are kept premultiplied (y by dx, z by dx*dy), so a row costs no integer
+// division or multiplication.
+__device__ __forceinline__ int next_axis_base(int base, int c, int stride, int limit)
+{
+    return c + kOrigin <= 0 ? 0 : (base + stride == limit ? 0 : base + stride);
+}
+
 struct QueryRows {
     int hx0, seg1, seg2;     // x run: [hx0, hx0+seg1) then [0, seg2)
     int ys, ny, zs, nz;      // virtual y/z start and extent
     int yi, zi, part;        // current row
-    int row_base;
+    int yb0, yb, zb;         // hash_axis(ys, dy) * dx, and the current row's y and z bases
     int k, kend;             // current slot range
 
     __device__ __forceinline__ void begin(const GridView& g, float qx, float qy, float qz, float r)
@@ -279,13 +289,14 @@ struct QueryRows {
         part = 0;
         k = 0;
         kend = 0;
-        row_base = 0;
+        yb0 = hash_axis(ys, g.dy) * g.dx;
+        yb = yb0;
+        zb = hash_axis(zs, g.dz) * g.dxy;
         if (nz > 0) load_row(g);
     }
     __device__ __forceinline__ void load_row(const GridView& g)
     {
-        row_base = g.base + hash_axis(zs + zi, g.dz) * g.dxy + hash_axis(ys + yi, g.dy) * g.dx;
-        int a = row_base + (part == 0 ? hx0 : 0);
+        int a = g.base + zb + yb + (part == 0 ? hx0 : 0);
         int b = a + (part == 0 ? seg1 : seg2);
         k = min(max(g.start[a], 0), g.npts);
         kend = min(max(g.start[b], k), g.npts);
@@ -299,13 +310,24 @@ struct QueryRows {
             part = 0;
             if (++yi == ny) {
                 yi = 0;
+                yb = yb0;
                 if (++zi == nz) return false;
+                zb = next_axis_base(zb, zs + zi, g.dxy, g.dxy * g.dz);
+            } else {
+                yb = next_axis_base(yb, ys + yi, g.dx, g.dxy);
             }
         }
         load_row(g);
         return true;
     }
     __device__ __forceinline__ bool valid() const { return nz > 0; }
+    // Advance past empty rows; false when the query is exhausted.
+    __device__ __forceinline__ bool skip_empty(const GridView& g)
+    {
+        while (k >= kend)
+            if (!advance(g)) return false;
+        return true;
+    }
 };
 
 // Per-particle neighbour list: the within-cut candidates of a query in visit
@@ -414,7 +436,11 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
     int candidates = 0;
     // entry `written` of slot s; stepped by n so that cap * n may pass 2^31
     int* __restrict__ out = L.idx + s;
-    if (it.valid()) {
+    // Every trip of the batch loop below tests one batch: the current range is
+    // kept non-empty, and a lane whose batch ends its row moves to the next
+    // row in the same trip (the bucket-range loads of that row are issued
+    // before the batch's tests, so both round trips overlap).
+    if (it.valid() && it.skip_empty(g)) {
         int* list = lds + threadIdx.x;
         bool done = false;
         while (true) {
@@ -422,15 +448,14 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
             while (!done && cnt <= CAP - kBatch) {
                 const int k0 = it.k;
                 const int rem = it.kend - k0;
-                if (rem <= 0) {
-                    if (!it.advance(g)) done = true;
-                    continue;
-                }
                 const int take = rem < kBatch ? rem : kBatch;
                 float4 q[kBatch];
+                const bool row_end = take == rem;
                 if constexpr (LEAN) {
 #pragma unroll
                     for (int u = 0; u < kBatch; ++u) q[u] = pos[k0 + (u < take ? u : take - 1)];
+                    it.k = k0 + take;
+                    if (row_end && !it.advance(g)) done = true;
 #pragma unroll
                     for (int u = 0; u < kBatch; ++u) {
                         if (u < take) {
@@ -445,6 +470,8 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
                     const int off = (int)((unsigned)k0 * 16u);
 #pragma unroll
                     for (int u = 0; u < kBatch; ++u) q[u] = ld_f4(rs, off, u * 16);
+                    it.k = k0 + take;
+                    if (row_end && !it.advance(g)) done = true;
 #pragma unroll
                     for (int u = 0; u < kBatch; ++u) {
                         if (kCounts && u < take) count(k0 + u);
@@ -454,8 +481,8 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
                         cnt += (u < take && !(d2 > keep2)) ? 1 : 0;
                     }
                 }
-                it.k = k0 + take;
                 candidates += take;
+                if (row_end && !done && !it.skip_empty(g)) done = true;
             }
             auto rec = [&](int k) {
                 if (written < L.cap) {
