@@ -330,8 +330,9 @@ struct QueryRows {
     }
 };
 
-// Per-particle neighbour list: the within-cut candidates of a query in visit
-// order, entry e of sorted slot s at idx[e * n + s] (coalesced across lanes).
+// Per-particle neighbour list: the active within-cut candidates of a query in
+// visit order, entry e of sorted slot s at idx[e * n + s] (coalesced across
+// lanes).
 // cnt[s] > cap marks an overflowed list; its users fall back to scanning.
 struct NbrList {
     int* idx;
@@ -403,6 +404,23 @@ __device__ __forceinline__ void run_body(Body& body, int k)
     if constexpr (IsPhased<typename std::decay<Body>::type>::value) body.body(k, body.load(k));
     else body(k);
 }
+// Data of a neighbour taken from a recorded list.  Lists hold active
+// neighbours only (the scanning loop records a survivor after its flags are
+// loaded), so the flags are known and their load is dropped; every body
+// tests no flag of a neighbour but kActive.
+template <class Body>
+__device__ __forceinline__ auto load_listed(Body& body, int k)
+{
+    auto d = body.load(k);
+    d.flags = kActive;
+    return d;
+}
+template <class Body>
+__device__ __forceinline__ void run_listed(Body& body, int k)
+{
+    if constexpr (IsPhased<typename std::decay<Body>::type>::value) body.body(k, load_listed(body, k));
+    else body(k);
+}
 constexpr int kPhasedUnroll = 4;   // neighbours whose data is fetched together
 
 // Visit the candidates of hash_grid_query(p, r) in Warp order (returning how
@@ -413,7 +431,9 @@ constexpr int kPhasedUnroll = 4;   // neighbours whose data is fetched together
 // survivors are compacted through a per-lane LDS list of CAP entries so that
 // the expensive body runs on full waves.  `count(k)` is called for every
 // candidate (DFSPH's unfiltered neighbour count).  With a kListBuild list the
-// survivors (radius^2 <= L.cut2) are also recorded for later passes.
+// active survivors (radius^2 <= L.cut2) are also recorded for later passes:
+// every pass skips inactive neighbours, so leaving them out of the lists
+// changes no result.
 //
 // The primary form loads through a buffer resource (32-bit offsets, an
 // immediate offset per candidate; slots past a row's end are read and
@@ -427,6 +447,9 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
                                                const NbrList& L, Count&& count, Body&& body)
 {
     static_assert(CAP >= 2 * kBatch, "list capacity");
+    // lists hold active neighbours only, which needs the flags of the phased
+    // payload at record time (the lean scan never records)
+    static_assert(LEAN || IsPhased<typename std::decay<Body>::type>::value, "scanning bodies must be phased");
     constexpr bool kCounts = !__is_same(typename std::decay<Count>::type, NoCount);
     const bool record = L.mode == kListBuild;
     const float keep2 = record ? fmax_(L.cut2, cut2) : cut2;
@@ -497,21 +520,27 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
                     int k[kPhasedUnroll];
 #pragma unroll
                     for (int u = 0; u < kPhasedUnroll; ++u) k[u] = list[(e + u) * BLOCK];
-                    if (record) {
-#pragma unroll
-                        for (int u = 0; u < kPhasedUnroll; ++u) rec(k[u]);
-                    }
                     decltype(body.load(0)) d[kPhasedUnroll];
 #pragma unroll
                     for (int u = 0; u < kPhasedUnroll; ++u) d[u] = body.load(k[u]);
+                    if (record) {
+#pragma unroll
+                        for (int u = 0; u < kPhasedUnroll; ++u)
+                            if (d[u].flags & kActive) rec(k[u]);
+                    }
 #pragma unroll
                     for (int u = 0; u < kPhasedUnroll; ++u) body.body(k[u], d[u]);
                 }
             }
             for (; e < cnt; ++e) {
                 const int k = list[e * BLOCK];
-                if (record) rec(k);
-                run_body(body, k);
+                if constexpr (IsPhased<typename std::decay<Body>::type>::value) {
+                    const auto d = body.load(k);
+                    if (record && (d.flags & kActive)) rec(k);
+                    body.body(k, d);
+                } else {
+                    run_body(body, k);
+                }
             }
             if (done) break;
         }
@@ -545,10 +574,10 @@ __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4
                     const int k2 = idx2[0], k3 = idx2[n];
                     idx = idx2 + 2 * (ptrdiff_t)n;
                     if constexpr (IsPhased<typename std::decay<Body>::type>::value) {
-                        const auto d0 = body.load(k0);
-                        const auto d1 = body.load(k1);
-                        const auto d2 = body.load(k2);
-                        const auto d3 = body.load(k3);
+                        const auto d0 = load_listed(body, k0);
+                        const auto d1 = load_listed(body, k1);
+                        const auto d2 = load_listed(body, k2);
+                        const auto d3 = load_listed(body, k3);
                         body.body(k0, d0);
                         body.body(k1, d1);
                         body.body(k2, d2);
@@ -560,7 +589,7 @@ __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4
                         body(k3);
                     }
                 }
-                for (; e < m; ++e, idx += n) run_body(body, *idx);
+                for (; e < m; ++e, idx += n) run_listed(body, *idx);
                 return 0;
             }
         }
