@@ -122,12 +122,10 @@ struct HashArgs {
     int* count;
 };
 
-// Position of point i after the solver's pre-op; writes the pre-op outputs
-// and the bucket id, returns the bucket.
+// Position of point i after the solver's pre-op; writes the pre-op outputs.
 template <int MODE>
-__device__ __forceinline__ int hash_point(const HashArgs& a, int i)
+__device__ __forceinline__ v3 pre_point(const PreOp& p, int i)
 {
-    const PreOp& p = a.pre;
     v3 x = ld3(p.x_in, i);
     if (MODE == kPreIntegrate) {
         // integrate_positions  solver_dfsph.py:340-353
@@ -159,6 +157,26 @@ __device__ __forceinline__ int hash_point(const HashArgs& a, int i)
         st3(p.v_out, i, v);
         st3(p.x_out, i, x);
     }
+    return x;
+}
+
+// The pre-op alone, in original order (coalesced): used when the hash kernel
+// visits points in the previous build's sorted order, where the pre-op's
+// loads and stores of a wave would each touch a separate cache line.
+template <int MODE>
+__global__ __launch_bounds__(kElemBlock) void k_pre_op(PreOp p, int n)
+{
+    const int i = blockIdx.x * kElemBlock + threadIdx.x;
+    if (i < n) pre_point<MODE>(p, i);
+}
+
+// Pre-op (when fused), bucket id of point i; writes the bucket id and
+// returns it.
+template <int MODE>
+__device__ __forceinline__ int hash_point(const HashArgs& a, int i)
+{
+    const PreOp& p = a.pre;
+    const v3 x = pre_point<MODE>(p, i);
     int c;
     if (p.scenes) {
         // segmented grid: the point's scene selects cell width, dims and bucket base
@@ -467,9 +485,16 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
     const int* order = reorder ? g->point_ids : nullptr;
     HashArgs ha{gv, pre, n, order, g->cell_of, g->rank, g->cell_count};
     const int nb = blocks_for(n, kElemBlock);
+    if (order && nb > 0 && pre.mode != kPrePlain) {
+        // pre-op in original order first; the hash then buckets its output
+        if (pre.mode == kPreIntegrate) FS_LAUNCH(k_pre_op<kPreIntegrate>, dim3(nb), dim3(kElemBlock), 0, s, pre, n);
+        else FS_LAUNCH(k_pre_op<kPrePredict>, dim3(nb), dim3(kElemBlock), 0, s, pre, n);
+        ha.pre.mode = kPrePlain;
+        ha.pre.x_in = pre.x_out;
+    }
     if (nb > 0 || pre.control_reset) {
         const int nbh = nb > 0 ? nb : 1;
-        switch (pre.mode) {
+        switch (ha.pre.mode) {
         case kPreIntegrate: FS_LAUNCH(k_hash_count<kPreIntegrate>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
         case kPrePredict: FS_LAUNCH(k_hash_count<kPrePredict>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
         default: FS_LAUNCH(k_hash_count<kPrePlain>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
