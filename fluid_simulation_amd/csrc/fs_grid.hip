@@ -192,7 +192,6 @@ __device__ __forceinline__ int hash_point(const HashArgs& a, int i)
     } else {
         c = bucket_of(a.g, x.x, x.y, x.z);
     }
-    a.cell_of[i] = c;
     return c;
 }
 
@@ -229,7 +228,12 @@ __global__ __launch_bounds__(kElemBlock) void k_hash_count(HashArgs a)
     const unsigned long long mine = heads & below;
     const int lead = mine ? 63 - __clzll((long long)mine) : 0;
     const int lead_base = __shfl(base, lead, 64);
-    if (valid) a.rank[i] = lead_base + (lane - lead);
+    // bucket and rank in processing order: coalesced stores (k_scatter maps
+    // position t back to its point)
+    if (valid) {
+        a.cell_of[t] = c;
+        a.rank[t] = lead_base + (lane - lead);
+    }
 }
 
 
@@ -357,18 +361,20 @@ __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restric
     if (tile == ntiles - 1 && t == 0) out[m] = s_excl + agg;
 }
 
-__global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __restrict__ cell_of,
-                                                        const int* __restrict__ rank, const int* __restrict__ start,
-                                                        int2* __restrict__ scatter, int* __restrict__ count,
-                                                        uint64_t* scan_desc_clear, int ntiles)
+__global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __restrict__ order,
+                                                        const int* __restrict__ cell_t, const int* __restrict__ rank_t,
+                                                        const int* __restrict__ start, int2* __restrict__ scatter,
+                                                        int* __restrict__ count, uint64_t* scan_desc_clear, int ntiles)
 {
-    const int i = blockIdx.x * kElemBlock + threadIdx.x;
-    if (i < ntiles) scan_desc_clear[i] = 0;
-    if (i >= n) return;
-    int c = cell_of[i];
-    int slot = start[c] + rank[i];
+    const int t = blockIdx.x * kElemBlock + threadIdx.x;
+    if (t < ntiles) scan_desc_clear[t] = 0;
+    if (t >= n) return;
+    // position t of the hash's processing order holds point order[t] (or t)
+    const int i = order ? order[t] : t;
+    const int c = cell_t[t];
+    const int slot = start[c] + rank_t[t];
     // (index, bucket): the fix-up reads the bucket with the index instead of
-    // a dependent cell_of[i] load
+    // a dependent bucket load
     if ((unsigned)slot < (unsigned)n) scatter[slot] = make_int2(i, c);
     count[c] = 0;
 }
@@ -382,32 +388,37 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __
     const int2 e = scatter[k];
     const int i = e.x;
     const int c = e.y;
+    // the point's gathered data depends only on i: its loads are issued
+    // before the rank loop, whose latency then covers theirs
+    v3 x = V(0.0f, 0.0f, 0.0f), u = V(0.0f, 0.0f, 0.0f);
+    float m = 0.0f, rd = 0.0f;
+    int fl = 0, mk = 0, sc = 0;
+    if (ga.pos) {
+        x = ld3(ga.x, i);
+        if (ga.mass) m = ga.mass[i];
+        if (ga.wvol) rd = ga.rest_density[i];
+        fl = ga.flags[i];
+        if (ga.mask) mk = ga.mask[i];
+        if (ga.scene_of) sc = ga.scene_of[i];
+        if (ga.v) u = ld3(ga.v, i);
+    }
     const int a = min(max(start[c], 0), n), b = min(max(start[c + 1], a), n);
     int r = 0;
-    for (int m = a; m < b; ++m) r += (scatter[m].x < i) ? 1 : 0;
+    for (int j = a; j < b; ++j) r += (scatter[j].x < i) ? 1 : 0;
     const int f = a + r;
     point_ids[f] = i;
     point_cells[f] = c;
     if (ga.pos) {
-        v3 x = ld3(ga.x, i);
-        float m = ga.mass ? ga.mass[i] : 0.0f;
         ga.pos[f] = make_float4(x.x, x.y, x.z, m);
-        if (ga.wvol) {
-            float rd = ga.rest_density[i];
-            // compute_particle_volume (solver_wcsph.py:109-116) times rest density,
-            // as compute_density forms mass_nei (solver_wcsph.py:148)
-            ga.wvol[f] = rd * (m / fmax_(rd, 1.0e-6f));
-        }
-        int fl = ga.flags[i];
-        int act = fl & 1;
-        ga.sflags[f] = act | ((act && (!ga.mask || ga.mask[i] == 0)) ? kDynamic : 0);
+        // compute_particle_volume (solver_wcsph.py:109-116) times rest density,
+        // as compute_density forms mass_nei (solver_wcsph.py:148)
+        if (ga.wvol) ga.wvol[f] = rd * (m / fmax_(rd, 1.0e-6f));
+        const int act = fl & 1;
+        ga.sflags[f] = act | ((act && mk == 0) ? kDynamic : 0);
         if (ga.any_inactive && __ballot(!act) && (threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1)
             atomicOr(ga.any_inactive, 1);
-        if (ga.scene_of) ga.sscene[f] = ga.scene_of[i];
-        if (ga.v) {
-            v3 u = ld3(ga.v, i);
-            ga.vel[f] = make_float4(u.x, u.y, u.z, 0.0f);
-        }
+        if (ga.scene_of) ga.sscene[f] = sc;
+        if (ga.v) ga.vel[f] = make_float4(u.x, u.y, u.z, 0.0f);
     }
 }
 
@@ -504,7 +515,7 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
     FS_LAUNCH(k_bucket_scan, dim3(ntiles), dim3(kScanBlock), 0, s, g->cell_count, g->cell_start, m, ntiles,
               g->scan_state);
     const int nbs = nb > blocks_for(ntiles, kElemBlock) ? nb : blocks_for(ntiles, kElemBlock);
-    FS_LAUNCH(k_scatter, dim3(nbs), dim3(kElemBlock), 0, s, n, g->cell_of, g->rank, g->cell_start,
+    FS_LAUNCH(k_scatter, dim3(nbs), dim3(kElemBlock), 0, s, n, order, g->cell_of, g->rank, g->cell_start,
               reinterpret_cast<int2*>(g->scatter_ids),
               g->cell_count, g->scan_state, ntiles);
     if (nb == 0) return FS_OK;

@@ -3,8 +3,9 @@
 ``HashGrid(dim_x, dim_y, dim_z)`` / ``reserve(n)`` / ``build(points, radius)``
 mirror the Warp calls the reference makes (``solver_wcsph.py:398-402,456``).
 After a build, ``point_ids`` (sorted slot -> original index), ``point_cells``
-(sorted bucket keys), ``cell_of`` (bucket of each original point) and
-``cell_tables()`` (Warp-layout start/end tables) are zero-copy torch views.
+(sorted bucket keys) and the bucket prefix ``cell_start`` are zero-copy torch
+views; ``cell_of`` (bucket of each original point) and ``cell_tables()``
+(Warp-layout start/end tables) are derived from them.
 """
 from __future__ import annotations
 
@@ -74,7 +75,11 @@ class HashGrid:
 
     @property
     def cell_of(self) -> torch.Tensor:
-        return self._view(self._c.cell_of, self.num_points)
+        """Bucket of each original point, from the sorted arrays (the build
+        itself keeps buckets only in its processing order)."""
+        out = torch.empty(self.num_points, dtype=torch.int32, device=self.device)
+        out[self.point_ids.long()] = self.point_cells
+        return out
 
     @property
     def cell_start(self) -> torch.Tensor:

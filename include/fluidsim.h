@@ -97,10 +97,15 @@ typedef struct fs_grid {
     float cell_width_inv;   /* 1/cell_width in float32                         */
     int32_t* point_ids;     /* [capacity] sorted slot -> original index        */
     int32_t* point_cells;   /* [capacity] sorted bucket keys                   */
-    int32_t* cell_of;       /* [capacity] bucket of original point i           */
+    int32_t* cell_of;       /* [capacity] scratch: bucket of the point at each
+                               position of the build's processing order (the
+                               previous build's sorted order when it covered
+                               the same points, else original order); the
+                               bucket of original point point_ids[s] is
+                               point_cells[s]                                   */
     int32_t* cell_start;    /* [dims+1] exclusive prefix over buckets           */
     int32_t* cell_count;    /* [dims] per-bucket counters, zero between builds  */
-    int32_t* rank;          /* [capacity] scratch                              */
+    int32_t* rank;          /* [capacity] scratch, in the same order as cell_of */
     int32_t* scatter_ids;   /* [2*capacity] scratch: (index, bucket) pairs     */
     uint64_t* scan_state;   /* look-back descriptors of the bucket scan         */
     uint32_t* sync;         /* [8] self-resetting counters                      */
