@@ -5,14 +5,19 @@ neighbour pass of that solver) over the full dam-break scene
 (``make_scene(400000)`` -> 401,856 particles), inputs resident in HBM.  The
 headline ``value`` is WCSPH (the solver the reference publishes a substep rate
 for); DFSPH and PBF are measured the same way and reported under
-``solvers``.  dt follows the dam-break driver's policy, evaluated once on the
-device state before the timed region so the timed substeps run back to back
-without host synchronisation.
+``solvers``.  Each solver first runs ``--settle-frames`` frames (default 30,
+half a second of the collapse) with the driver's dt policy, untimed, so the
+timed substeps see the flow the reference's published 60-frame runs average
+over rather than the initial lattice at rest; the initial state's rate is
+kept under ``initial_state``.  dt follows the dam-break driver's policy,
+evaluated once on the device state before the timed region so the timed
+substeps run back to back without host synchronisation.
 
 ``roofline`` covers the dominant kernel of the WCSPH substep: algorithmic bytes
 per launch (the per-particle byte model below x 401,856) over its mean launch
 duration, from HIP events on the launch stream.  ``cpu_baseline`` times the
-oracle's FP32 restatement of the same WCSPH substep on the host cores.
+oracle's FP32 restatement of the same WCSPH substep on the host cores,
+starting from the state the timed GPU substeps start from.
 
 The default run also adds, under ``extras``: the 64-scene batched substep
 (graph-replayed, eager and 64 sequential solves), the differentiable WCSPH
@@ -27,7 +32,7 @@ the live launch duration, against one wave64 VALU instruction per CU per
 cycle.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--solvers wcsph,dfsph,pbf]
-       [--no-tune] [--no-extras] [--no-cpu-baseline]
+       [--settle-frames F] [--no-tune] [--no-extras] [--no-cpu-baseline]
 Other modes (each prints its own JSON lines and exits):
   --micro 4000000,16000000   grid build / density / force at 4M and 16M points,
                              uniform and clustered (median of >= 20 substeps)
@@ -120,6 +125,9 @@ def parse():
     ap.add_argument("--solvers", default="wcsph,dfsph,pbf")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-substeps", type=int, default=40)
+    ap.add_argument("--settle-frames", type=int, default=30,
+                    help="dam-break frames each solver runs (untimed, with the driver's dt policy) before the "
+                         "timed substeps; 0 times the initial state")
     ap.add_argument("--copy-calibration", action="store_true", help="run only the PMC copy calibration kernel")
     ap.add_argument("--micro", default="", help="comma-separated particle counts for the grid/density/force "
                                                 "microbenchmark (e.g. 4000000,16000000)")
@@ -144,13 +152,41 @@ def driver_dt(sim):
     return min(sim.frame_dt / sim.substeps, sim.cfl_limit(*sim._cfl_maxima()))
 
 
-def bench_solver(method, scene, steps, warmup, tune=True):
+def _time_substeps(sim, dt, steps):
+    import torch
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim._substep(dt)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def bench_solver(method, scene, steps, warmup, tune=True, settle_frames=0):
+    """Time `steps` substeps of `method` at the state `settle_frames` frames
+    into the dam break (the driver advances the run untimed first); the
+    initial state's rate is recorded too, under ``initial_state``."""
     import torch
 
     from fluid_simulation_amd import _lib
     from fluid_simulation_amd.examples.dam_break.simulation import DamBreakSimulation
 
     sim = DamBreakSimulation(scene, method, device="cuda:0")
+    n = scene.count
+    initial = None
+    if settle_frames > 0:
+        dt0 = driver_dt(sim)
+        for _ in range(warmup):
+            sim._substep(dt0)
+        el0 = _time_substeps(sim, dt0, steps)
+        initial = {"particle_substeps_per_s": n * steps / el0, "ms_per_substep": 1e3 * el0 / steps, "dt": dt0}
+        t_settle = time.perf_counter()
+        for _ in range(settle_frames):
+            sim.step()
+        torch.cuda.synchronize()
+        print(f"# {method}: advanced {settle_frames} frames in {time.perf_counter() - t_settle:.1f} s",
+              file=sys.stderr, flush=True)
     for _ in range(warmup):
         sim._substep(driver_dt(sim))
     dt = driver_dt(sim)
@@ -164,12 +200,12 @@ def bench_solver(method, scene, steps, warmup, tune=True):
                             ("wcsph_density", "wcsph_force"))
         print("# wcsph autotune: " + ", ".join(f"{k} -> block {v['block']} depth {v['list_depth']}"
                                                for k, v in tuned.items()), file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        sim._substep(dt)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    state = None
+    if method == "wcsph":
+        # the timed substeps start here: the CPU baseline restarts from this state
+        state = (sim.state_0.particle_q.tensor.cpu().numpy().copy(),
+                 sim.state_0.particle_qd.tensor.cpu().numpy().copy(), dt)
+    elapsed = _time_substeps(sim, dt, steps)
     iters = sim.solver.last_iterations() if method == "dfsph" else None
     # per-kernel timing: the same substeps again with HIP events around each launch
     lib = _lib.load()
@@ -184,9 +220,11 @@ def bench_solver(method, scene, steps, warmup, tune=True):
         c0, m0 = prof.get(key, (0, 0.0))
         prof[key] = (c0 + cnt, m0 + ms)
     sim.check_finite()
-    n = scene.count
     rate = n * steps / elapsed
-    out = {"particle_substeps_per_s": rate, "ms_per_substep": 1e3 * elapsed / steps, "dt": dt}
+    out = {"particle_substeps_per_s": rate, "ms_per_substep": 1e3 * elapsed / steps, "dt": dt,
+           "frames_before_timing": settle_frames, "sim_time_s": sim.sim_time}
+    if initial:
+        out["initial_state"] = initial
     if method == "wcsph":
         out["kernels_ms_per_substep"] = {k: v[1] / steps for k, v in sorted(prof.items())}
     else:
@@ -205,10 +243,13 @@ def bench_solver(method, scene, steps, warmup, tune=True):
     out["bytes_per_particle_substep"] = b
     out["effective_GBs"] = b * rate / 1e9
     out["hbm_frac"] = out["effective_GBs"] / HBM_PEAK_GBS
-    return out, prof
+    return out, prof, state
 
 
-def cpu_baseline(scene, substeps, threads=None):
+def cpu_baseline(scene, substeps, threads=None, state=None):
+    """The oracle's FP32 WCSPH substep on the host cores, from `state`
+    ((x, v, dt) of the GPU run where its timed substeps start) or from the
+    initial scene at dt = 1e-4."""
     import numpy as np
 
     from oracle import oracle as O
@@ -223,16 +264,21 @@ def cpu_baseline(scene, substeps, threads=None):
                         viscosity=mat["viscosity"], surface_tension=mat["surface_tension"],
                         gravity=scene.spec.gravity, radius=scene.radius)
     m.world[:] = -1
-    st = O.OState(scene.positions.copy(), np.zeros_like(scene.positions))
+    if state is not None:
+        x0, v0, dt = state
+        st = O.OState(np.ascontiguousarray(x0, dtype=np.float32), np.ascontiguousarray(v0, dtype=np.float32))
+    else:
+        st = O.OState(scene.positions.copy(), np.zeros_like(scene.positions))
+        dt = 1.0e-4
     ocfg = O.WcsphConfig(**{k: getattr(cfg, k) for k in O.WcsphConfig.__dataclass_fields__})
-    dt = 1.0e-4
     O.wcsph_step(m, st, ocfg, dt)   # warm-up (page-in, thread pool)
     t0 = time.perf_counter()
     for _ in range(substeps):
         st, _ = O.wcsph_step(m, st, ocfg, dt)
     el = time.perf_counter() - t0
+    where = "from the GPU run's timed state" if state is not None else "from the initial state"
     return {"value": n * substeps / el, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
-            "sample": f"{substeps} WCSPH substeps of the {n}-particle dam break, FP32 C oracle, OpenMP "
+            "sample": f"{substeps} WCSPH substeps of the {n}-particle dam break {where}, FP32 C oracle, OpenMP "
                       f"{threads} threads, {el:.1f} s"}
 
 
@@ -703,13 +749,16 @@ def main():
     scene = make_scene(args.particles)
     results = {}
     prof_wcsph = None
+    wcsph_state = None
     for method in [s for s in args.solvers.split(",") if s]:
-        print(f"# bench {method}: n={scene.count} steps={args.steps} warmup={args.warmup}", file=sys.stderr,
-              flush=True)
-        res, prof = bench_solver(method, scene, args.steps, args.warmup, tune=not args.no_tune)
+        print(f"# bench {method}: n={scene.count} steps={args.steps} warmup={args.warmup} "
+              f"after {args.settle_frames} frames", file=sys.stderr, flush=True)
+        res, prof, state = bench_solver(method, scene, args.steps, args.warmup, tune=not args.no_tune,
+                                        settle_frames=args.settle_frames)
         results[method] = res
         if method == "wcsph":
             prof_wcsph = prof
+            wcsph_state = state
         print(f"# {method}: {res['particle_substeps_per_s'] / 1e6:.1f} M particle-substeps/s, "
               f"{res['ms_per_substep']:.3f} ms/substep", file=sys.stderr, flush=True)
     head = "wcsph" if "wcsph" in results else next(iter(results))
@@ -751,7 +800,7 @@ def main():
     cpu = cpu_detail = None
     if not args.no_cpu_baseline:
         print("# cpu baseline (oracle)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(scene, args.cpu_substeps)
+        cpu = cpu_baseline(scene, args.cpu_substeps, state=wcsph_state)
         if args.particles == 400000:
             cpu_detail = cpu_baseline_detail(scene)
     value = headline["particle_substeps_per_s"]
@@ -767,9 +816,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": value / PUBLISHED_WCSPH if head == "wcsph" else None,
         "dtype": "f32",
-        "data": "synthetic: make_scene(400000) dam-break lattice (401,856 particles), zero initial velocity",
-        "config": {"workload": f"{head} dam break, {scene.count} particles, one solver substep per step",
-                   "particles": scene.count, "solver": head, "dt": headline["dt"]},
+        "data": "synthetic: make_scene(400000) dam-break lattice (401,856 particles), zero initial velocity, "
+                f"advanced {args.settle_frames} frames by the solver before the timed substeps",
+        "config": {"workload": f"{head} dam break, {scene.count} particles, one solver substep per step, "
+                               f"timed at the state {args.settle_frames} frames "
+                               f"({args.settle_frames / 60:.2f} s) into the run",
+                   "particles": scene.count, "solver": head, "frames_before_timing": args.settle_frames,
+                   "dt": headline["dt"]},
         "hbm_frac_substep": headline["hbm_frac"],
         "roofline": roofline,
         "valu_issue": valu,

@@ -1300,7 +1300,7 @@ struct Variant {
 };
 // variant 0 is the default
 constexpr Variant kScanVariants[] = {{64, 48},  {64, 24},  {64, 32},  {64, 64},  {128, 24}, {128, 32},
-                                     {128, 16}, {256, 32}, {256, 24}, {256, 16}};
+                                     {128, 16}, {256, 32}, {256, 24}, {256, 16}, {64, 40},  {64, 56}};
 constexpr Variant kReplayVariants[] = {{64, 16}, {128, 16}, {256, 16}, {512, 16}};
 constexpr int kNumScan = sizeof(kScanVariants) / sizeof(kScanVariants[0]);
 constexpr int kNumReplay = sizeof(kReplayVariants) / sizeof(kReplayVariants[0]);

@@ -39,9 +39,10 @@ class ParticleSolver(SolverBase):
 
     #: neighbour-list entries reserved per particle (lists replayed by the
     #: passes that follow a grid build; overflowing particles scan the grid).
-    #: 96 covers the dam break's ~34 neighbours with room for compression;
-    #: denser inputs pass a larger ``neighbor_list_capacity`` (keyword-only,
-    #: after the reference's (model, config)).
+    #: 96 covers the WCSPH dam break's 33 neighbours at rest and ~40 in its
+    #: splash; DFSPH and PBF set their own.  Denser inputs pass a larger
+    #: ``neighbor_list_capacity`` (keyword-only, after the reference's
+    #: (model, config)).
     neighbor_list_capacity = 96
 
     def __init__(self, model, config, *, neighbor_list_capacity: int | None = None):

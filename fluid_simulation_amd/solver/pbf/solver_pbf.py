@@ -34,6 +34,8 @@ def poly6(r_norm: float, h: float) -> float:
 
 class SolverPBF(ParticleSolver):
     _name = "SolverPBF"
+    #: the PBF dam break reaches ~80 neighbours within h in its splash
+    neighbor_list_capacity = 128
 
     @dataclass
     class PbfConfig:

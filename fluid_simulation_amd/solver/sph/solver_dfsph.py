@@ -24,6 +24,10 @@ from .._base import ParticleSolver, ptr, register, register_common_model_attribu
 
 class SolverDFSPH(ParticleSolver):
     _name = "SolverDFSPH"
+    #: the dam break's DFSPH config (max kappa 1.0) lets the flow compress
+    #: into clumps of up to ~270 neighbours within h; an overflowing particle
+    #: rescans the grid in every replaying pass, so the lists are deep
+    neighbor_list_capacity = 384
 
     @dataclass
     class SphConfig:
