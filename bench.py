@@ -163,6 +163,19 @@ def _time_substeps(sim, dt, steps):
     return time.perf_counter() - t0
 
 
+def _state_tensors(sim):
+    """Every device array the next substep reads: the current state (positions,
+    velocities, sph attributes) and the solver's stored acceleration."""
+    from fluid_simulation_amd.array import Array
+
+    st = sim.state_0
+    arrays = [st.particle_q, st.particle_qd, sim.solver._particle_accel]
+    sph = getattr(st, "sph", None)
+    if sph is not None:
+        arrays += [v for v in vars(sph).values() if isinstance(v, Array)]
+    return [a.tensor for a in arrays if a is not None]
+
+
 def bench_solver(method, scene, steps, warmup, tune=True, settle_frames=0):
     """Time `steps` substeps of `method` at the state `settle_frames` frames
     into the dam break (the driver advances the run untimed first); the
@@ -176,11 +189,19 @@ def bench_solver(method, scene, steps, warmup, tune=True, settle_frames=0):
     n = scene.count
     initial = None
     if settle_frames > 0:
+        # time the initial state, then put it back so that the settling run
+        # starts from the scene as given
+        saved = [t.clone() for t in _state_tensors(sim)]
+        substeps0 = sim.total_substeps
         dt0 = driver_dt(sim)
         for _ in range(warmup):
             sim._substep(dt0)
         el0 = _time_substeps(sim, dt0, steps)
         initial = {"particle_substeps_per_s": n * steps / el0, "ms_per_substep": 1e3 * el0 / steps, "dt": dt0}
+        for t, c in zip(_state_tensors(sim), saved):
+            t.copy_(c)
+        sim.total_substeps = substeps0
+        del saved
         t_settle = time.perf_counter()
         for _ in range(settle_frames):
             sim.step()
