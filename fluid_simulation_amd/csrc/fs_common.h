@@ -566,23 +566,45 @@ __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4
                 // pointer stepping keeps the addressing 64-bit (cap * n may
                 // pass 2^31)
                 const int* __restrict__ idx = L.idx + s;
-                const int n = L.n;
+                const ptrdiff_t n = L.n;
                 int e = 0;
-                for (; e + 4 <= m; e += 4) {
-                    const int k0 = idx[0], k1 = idx[n];
-                    const int* __restrict__ idx2 = idx + 2 * (ptrdiff_t)n;
-                    const int k2 = idx2[0], k3 = idx2[n];
-                    idx = idx2 + 2 * (ptrdiff_t)n;
-                    if constexpr (IsPhased<typename std::decay<Body>::type>::value) {
-                        const auto d0 = load_listed(body, k0);
-                        const auto d1 = load_listed(body, k1);
-                        const auto d2 = load_listed(body, k2);
-                        const auto d3 = load_listed(body, k3);
-                        body.body(k0, d0);
-                        body.body(k1, d1);
-                        body.body(k2, d2);
-                        body.body(k3, d3);
-                    } else {
+                if constexpr (IsPhased<typename std::decay<Body>::type>::value) {
+                    // software-pipelined by one group: the next four indices
+                    // are issued before this group's neighbour data, so a
+                    // group costs one round trip instead of two (long lists
+                    // are latency chains)
+                    if (m >= 4) {
+                        int k0 = idx[0], k1 = idx[n], k2 = idx[2 * n], k3 = idx[3 * n];
+                        idx += 4 * n;
+                        for (e = 4;; e += 4) {
+                            const bool more = e + 4 <= m;
+                            int n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+                            if (more) {
+                                n0 = idx[0];
+                                n1 = idx[n];
+                                n2 = idx[2 * n];
+                                n3 = idx[3 * n];
+                                idx += 4 * n;
+                            }
+                            const auto d0 = load_listed(body, k0);
+                            const auto d1 = load_listed(body, k1);
+                            const auto d2 = load_listed(body, k2);
+                            const auto d3 = load_listed(body, k3);
+                            body.body(k0, d0);
+                            body.body(k1, d1);
+                            body.body(k2, d2);
+                            body.body(k3, d3);
+                            if (!more) break;
+                            k0 = n0;
+                            k1 = n1;
+                            k2 = n2;
+                            k3 = n3;
+                        }
+                    }
+                } else {
+                    for (; e + 4 <= m; e += 4) {
+                        const int k0 = idx[0], k1 = idx[n], k2 = idx[2 * n], k3 = idx[3 * n];
+                        idx += 4 * n;
                         body(k0);
                         body(k1);
                         body(k2);
