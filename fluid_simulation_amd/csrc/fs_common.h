@@ -427,7 +427,8 @@ constexpr int kPhasedUnroll = 4;   // neighbours whose data is fetched together
 // many there were), but call
 // `body(k)` (k = sorted slot) only for those with squared distance not above
 // `cut2` -- every skipped candidate has q > 1 and contributes exactly zero to
-// every kernel sum here.  Candidate positions are loaded kBatch at a time;
+// every kernel sum here.  Candidate positions are loaded BATCH at a time
+// (kBatch unless the pass sets it: short rows waste less with small batches);
 // survivors are compacted through a per-lane LDS list of CAP entries so that
 // the expensive body runs on full waves.  `count(k)` is called for every
 // candidate (DFSPH's unfiltered neighbour count).  With a kListBuild list the
@@ -441,12 +442,12 @@ constexpr int kPhasedUnroll = 4;   // neighbours whose data is fetched together
 // LEAN is the fallback of the list-replaying passes: global loads clamped to
 // the row and branchy compaction, which keep that rarely taken path from
 // raising the register count (and lowering the occupancy) of its kernel.
-template <int BLOCK, int CAP, bool LEAN = false, class Count, class Body>
+template <int BLOCK, int CAP, bool LEAN = false, int BATCH = kBatch, class Count, class Body>
 __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* __restrict__ pos, int s, float px,
                                                float py, float pz, float r, float cut2, int* __restrict__ lds,
                                                const NbrList& L, Count&& count, Body&& body)
 {
-    static_assert(CAP >= 2 * kBatch, "list capacity");
+    static_assert(CAP >= 2 * BATCH, "list capacity");
     // lists hold active neighbours only, which needs the flags of the phased
     // payload at record time (the lean scan never records)
     static_assert(LEAN || IsPhased<typename std::decay<Body>::type>::value, "scanning bodies must be phased");
@@ -468,19 +469,19 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
         bool done = false;
         while (true) {
             int cnt = 0;
-            while (!done && cnt <= CAP - kBatch) {
+            while (!done && cnt <= CAP - BATCH) {
                 const int k0 = it.k;
                 const int rem = it.kend - k0;
-                const int take = rem < kBatch ? rem : kBatch;
-                float4 q[kBatch];
+                const int take = rem < BATCH ? rem : BATCH;
+                float4 q[BATCH];
                 const bool row_end = take == rem;
                 if constexpr (LEAN) {
 #pragma unroll
-                    for (int u = 0; u < kBatch; ++u) q[u] = pos[k0 + (u < take ? u : take - 1)];
+                    for (int u = 0; u < BATCH; ++u) q[u] = pos[k0 + (u < take ? u : take - 1)];
                     it.k = k0 + take;
                     if (row_end && !it.advance(g)) done = true;
 #pragma unroll
-                    for (int u = 0; u < kBatch; ++u) {
+                    for (int u = 0; u < BATCH; ++u) {
                         if (u < take) {
                             count(k0 + u);
                             float ddx = px - q[u].x, ddy = py - q[u].y, ddz = pz - q[u].z;
@@ -492,11 +493,11 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
                     const __amdgpu_buffer_rsrc_t rs = float4_rsrc(pos, g.npts);
                     const int off = (int)((unsigned)k0 * 16u);
 #pragma unroll
-                    for (int u = 0; u < kBatch; ++u) q[u] = ld_f4(rs, off, u * 16);
+                    for (int u = 0; u < BATCH; ++u) q[u] = ld_f4(rs, off, u * 16);
                     it.k = k0 + take;
                     if (row_end && !it.advance(g)) done = true;
 #pragma unroll
-                    for (int u = 0; u < kBatch; ++u) {
+                    for (int u = 0; u < BATCH; ++u) {
                         if (kCounts && u < take) count(k0 + u);
                         float ddx = px - q[u].x, ddy = py - q[u].y, ddz = pz - q[u].z;
                         float d2 = prefilter_d2(ddx, ddy, ddz);
@@ -554,7 +555,7 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
 // no list or it overflowed (the lean scan); a scanning pass always scans with
 // the primary loop and records when its list is in build mode.  The role is
 // a template parameter so that each kernel carries only the loop it runs.
-template <int BLOCK, int CAP, bool REPLAY, class Count, class Body>
+template <int BLOCK, int CAP, bool REPLAY, int BATCH = kBatch, class Count, class Body>
 __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4* __restrict__ pos, int s, float px,
                                                   float py, float pz, float r, float cut2, int* __restrict__ lds,
                                                   const NbrList& L, Count&& count, Body&& body)
@@ -617,11 +618,11 @@ __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4
         }
         NbrList none = L;
         none.mode = kListNone;
-        return scan_neighbors<BLOCK, CAP, true>(g, pos, s, px, py, pz, r, cut2, lds, none, count, body);
+        return scan_neighbors<BLOCK, CAP, true, BATCH>(g, pos, s, px, py, pz, r, cut2, lds, none, count, body);
     } else {
         NbrList scan = L;
         if (scan.mode == kListUse) scan.mode = kListNone;
-        return scan_neighbors<BLOCK, CAP, false>(g, pos, s, px, py, pz, r, cut2, lds, scan, count, body);
+        return scan_neighbors<BLOCK, CAP, false, BATCH>(g, pos, s, px, py, pz, r, cut2, lds, scan, count, body);
     }
 }
 

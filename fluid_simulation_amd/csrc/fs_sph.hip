@@ -1259,7 +1259,9 @@ __global__ __launch_bounds__(BLOCK) void k_separation(SepArgs a)
         const v3 xi = xyz(c.pos[s]);
         const float rest = a.rest_dist, strength = a.strength;
         float count = 0.0f;
-        SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+        // cells of the rest distance hold about one particle, so a row has
+        // about three candidates: batches of four waste less than eight
+        for_each_neighbor<BLOCK, CAP, false, 4>(c.g, c.pos, s, xi.x, xi.y, xi.z, c.h, c.cut2, lds, c.L, NoCount{}, phased(
             [&](int k) { return NbrX{c.sflags[k], c.pos[k]}; },
             [&](int k, const NbrX& nb) {
                 if (k == s || !(nb.flags & kActive)) return;
