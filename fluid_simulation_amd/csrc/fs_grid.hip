@@ -1,10 +1,17 @@
 // fs_grid.hip -- Warp-compatible uniform hash grid for gfx950.
 //
-// Build = four kernels on one stream, no host sync, no memset:
-//   1. k_hash_count   per original particle: bucket id, rank = atomicAdd(count)
+// Build = four kernels (five with a pre-op) on one stream, no host sync, no
+// memset:
+//   0. k_pre_op       (DFSPH/PBF) the solver's position update before the
+//                     build, in original order, when the hash runs in the
+//                     previous sorted order
+//   1. k_hash_count   per particle, in the previous build's sorted order:
+//                     bucket id, rank = atomicAdd(count), both stored at the
+//                     processing position
 //   2. k_bucket_scan  single-pass decoupled look-back exclusive scan of the
 //                     dense bucket counts -> cell_start[dims+1]
-//   3. k_scatter      slot = cell_start[b] + rank; re-zero the counter
+//   3. k_scatter      slot = cell_start[b] + rank for the point at each
+//                     processing position; re-zero the counter
 //   4. k_fix_gather   order each bucket's members by original index (the
 //                     stable (bucket, index) permutation of Warp's build),
 //                     and gather particle data into that sorted order.
