@@ -42,6 +42,7 @@ class FsWork(ctypes.Structure):
         ("pos", c_void_p), ("pos_alt", c_void_p), ("vel", c_void_p), ("sca", c_void_p), ("wvol", c_void_p),
         ("flags", c_void_p), ("x_pred", c_void_p), ("v_pred", c_void_p), ("nbr", c_void_p),
         ("nbr_count", c_void_p), ("partials", c_void_p), ("control", c_void_p), ("sync", c_void_p),
+        ("packed", c_void_p),
     ]
 
 

@@ -386,6 +386,17 @@ __global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __rest
     count[c] = 0;
 }
 
+// The per-point words the fix-up gathers besides the positions and
+// velocities, packed in original order (coalesced) into one record per point.
+__global__ __launch_bounds__(kElemBlock) void k_pack_static(int n, GatherOp ga)
+{
+    const int i = blockIdx.x * kElemBlock + threadIdx.x;
+    if (i >= n) return;
+    const float m = ga.mass ? ga.mass[i] : 0.0f;
+    const float rd = ga.wvol ? ga.rest_density[i] : 0.0f;
+    ga.packed[i] = make_int4(__float_as_int(m), __float_as_int(rd), ga.flags[i], ga.mask ? ga.mask[i] : 0);
+}
+
 __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __restrict__ scatter,
                                                            const int* __restrict__ start, int* __restrict__ point_ids,
                                                            int* __restrict__ point_cells, GatherOp ga)
@@ -402,10 +413,18 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __
     int fl = 0, mk = 0, sc = 0;
     if (ga.pos) {
         x = ld3(ga.x, i);
-        if (ga.mass) m = ga.mass[i];
-        if (ga.wvol) rd = ga.rest_density[i];
-        fl = ga.flags[i];
-        if (ga.mask) mk = ga.mask[i];
+        if (ga.packed) {
+            const int4 pk = ga.packed[i];
+            m = __int_as_float(pk.x);
+            rd = __int_as_float(pk.y);
+            fl = pk.z;
+            mk = pk.w;
+        } else {
+            if (ga.mass) m = ga.mass[i];
+            if (ga.wvol) rd = ga.rest_density[i];
+            fl = ga.flags[i];
+            if (ga.mask) mk = ga.mask[i];
+        }
         if (ga.scene_of) sc = ga.scene_of[i];
         if (ga.v) u = ld3(ga.v, i);
     }
@@ -526,6 +545,7 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
               reinterpret_cast<int2*>(g->scatter_ids),
               g->cell_count, g->scan_state, ntiles);
     if (nb == 0) return FS_OK;
+    if (ga.pos && ga.packed) FS_LAUNCH(k_pack_static, dim3(nb), dim3(kElemBlock), 0, s, n, ga);
     FS_LAUNCH(k_fix_gather, dim3(nb), dim3(kElemBlock), 0, s, n, reinterpret_cast<const int2*>(g->scatter_ids),
               g->cell_start, g->point_ids, g->point_cells, ga);
     return FS_OK;

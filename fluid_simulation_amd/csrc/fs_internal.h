@@ -49,6 +49,10 @@ struct GatherOp {
     const int* scene_of = nullptr; // optional: scene ids, gathered to sscene
     int* sscene = nullptr;
     int* any_inactive = nullptr;   // optional: set to 1 when some point is inactive
+    // optional scratch [n]: (mass, rest_density, flags, mask) of each point in
+    // original order, packed by the build before the fix-up so that the
+    // fix-up fetches one 16-byte record per point instead of four words
+    int4* packed = nullptr;
 };
 
 // Full build: hash (+pre-op), bucket scan, scatter, stable fix-up (+gather).

@@ -1408,6 +1408,7 @@ static GatherOp gather_for(const fs_model* m, fs_work* w, const float* x, const 
     ga.vel = vel_dst;
     ga.wvol = w->wvol;
     ga.sflags = w->flags;
+    ga.packed = reinterpret_cast<int4*>(w->packed);
     return ga;
 }
 
@@ -1424,7 +1425,7 @@ size_t fs_work_bytes(int32_t capacity, int32_t list_capacity)
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     size_t nblocks = (cap + kPassBlock - 1) / kPassBlock;
     return al(cap * 16) * 4 + al(cap * 4) * 2 + al(cap * 12) * 2 + al(cap * 4 * (size_t)list_capacity) +
-           al(cap * 4) + al(nblocks * 16 + 16) + al(kControlWords * 4) + al(8 * 4);
+           al(cap * 4) + al(nblocks * 16 + 16) + al(kControlWords * 4) + al(8 * 4) + al(cap * 16);
 }
 
 int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity, int32_t list_capacity,
@@ -1467,6 +1468,7 @@ int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity
     w->partials = (double*)take(nblocks * 16 + 16);
     w->control = (int32_t*)take(kControlWords * 4);
     w->sync = (uint32_t*)take(8 * 4);
+    w->packed = (int32_t*)take(cap * 16);
     hipStream_t s = (hipStream_t)stream;
     int32_t rc = hip_check(hipMemsetAsync(w->control, 0, kControlWords * 4, s), "fs_work_init memset");
     if (rc) return rc;

@@ -186,6 +186,9 @@ typedef struct fs_work {
     double* partials;   /* residual partial sums                               */
     int32_t* control;   /* [64] loop control: stop flags, iteration counters   */
     uint32_t* sync;     /* [8]  self-resetting counters                        */
+    int32_t* packed;    /* [cap*4] grid-build scratch, original order: mass,
+                           rest density, flags, mask of each particle packed
+                           into one 16-byte record (repacked every build)    */
 } fs_work;
 
 /* list_capacity: entries per particle of the neighbour lists recorded by the
