@@ -242,7 +242,9 @@ __device__ __forceinline__ int bucket_of(const GridView& g, float x, float y, fl
 
 // Iterator over the sorted-slot ranges visited by hash_grid_query(p, r):
 // rows (y,z) in Warp order; within a row the x run is one contiguous range
-// of sorted slots, or two when the run wraps past the last bucket.
+// of sorted slots, or two when the run wraps past the last bucket, and x
+// cells below -kOrigin (which hash to bucket 0 like -kOrigin itself) add one
+// range of bucket 0 each ahead of it.
 // The hashed y/z of successive rows are stepped, not recomputed: moving from
 // virtual coordinate c-1 to c, hash_axis adds one and wraps at the dimension,
 // except that coordinates at or below -kOrigin all hash to 0.  The row bases
@@ -255,6 +257,7 @@ __device__ __forceinline__ int next_axis_base(int base, int c, int stride, int l
 
 struct QueryRows {
     int hx0, seg1, seg2;     // x run: [hx0, hx0+seg1) then [0, seg2)
+    int xrep;                // one-bucket parts of bucket 0 ahead of the run (x cells below -kOrigin)
     int ys, ny, zs, nz;      // virtual y/z start and extent
     int yi, zi, part;        // current row
     int yb0, yb, zb;         // hash_axis(ys, dy) * dx, and the current row's y and z bases
@@ -276,6 +279,15 @@ struct QueryRows {
         nz = ze - zsv + 1;
         ys = ysv;
         zs = zsv;
+        // x cells at or below -kOrigin all hash to bucket 0, which the query
+        // then visits once per such cell; all but the last of them become
+        // one-bucket parts ahead of the run, which starts at bucket 0
+        xrep = 0;
+        if (lenx > 0 && xs < -kOrigin) {
+            xrep = min(xe, -kOrigin) - xs;
+            xs += xrep;
+            lenx -= xrep;
+        }
         hx0 = hash_axis(xs, g.dx);
         if (lenx <= 0 || ny <= 0 || nz <= 0) {
             nz = 0;
@@ -296,16 +308,16 @@ struct QueryRows {
     }
     __device__ __forceinline__ void load_row(const GridView& g)
     {
-        int a = g.base + zb + yb + (part == 0 ? hx0 : 0);
-        int b = a + (part == 0 ? seg1 : seg2);
+        int a = g.base + zb + yb + (part == xrep ? hx0 : 0);
+        int b = a + (part < xrep ? 1 : (part == xrep ? seg1 : seg2));
         k = min(max(g.start[a], 0), g.npts);
         kend = min(max(g.start[b], k), g.npts);
     }
     // Move to the next row; false when the query is exhausted.
     __device__ __forceinline__ bool advance(const GridView& g)
     {
-        if (part == 0 && seg2 > 0) {
-            part = 1;
+        if (part < xrep || (part == xrep && seg2 > 0)) {
+            ++part;
         } else {
             part = 0;
             if (++yi == ny) {

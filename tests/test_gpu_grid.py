@@ -34,10 +34,22 @@ def _points(kind):
         c = rng.uniform(0.2, 0.8, (4, 3))
         pts = (c[rng.integers(0, 4, 4000)] + rng.normal(0, 0.02, (4000, 3))).astype(np.float32)
         return pts, 0.02
+    if kind == "far_clamped":
+        # cell coordinates at and below -2^20, which all hash to bucket 0 on
+        # their axis: Warp's query then visits that bucket once per such cell
+        h = 0.02
+        edge = -(2 ** 20) * h
+        near = np.column_stack([rng.uniform(edge - 0.12, edge + 0.12, 600), rng.uniform(-0.05, 0.05, (600, 2))])
+        far_x = np.column_stack([rng.uniform(-2.2e4, -2.1e4, 300), rng.uniform(-0.05, 0.05, (300, 2))])
+        far_y = np.column_stack([rng.uniform(-0.05, 0.05, 200), rng.uniform(edge - 0.1, edge + 0.1, 200),
+                                 rng.uniform(-0.05, 0.05, 200)])
+        extreme = np.array([[-1.0e12, 0.0, 0.0], [-3.0e11, 0.01, 0.0]])
+        return np.vstack([near, far_x, far_y, extreme]).astype(np.float32), h
     raise KeyError(kind)
 
 
-KINDS = ["random_negative", "strip_aliasing", "on_faces", "dam_break_540", "dam_break_10k", "clustered"]
+KINDS = ["random_negative", "strip_aliasing", "on_faces", "dam_break_540", "dam_break_10k", "clustered",
+         "far_clamped"]
 
 
 def _gpu_grid(pts, h, round_mode=0):
