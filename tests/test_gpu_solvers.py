@@ -179,6 +179,29 @@ def test_list_capacity_does_not_change_results(method):
             assert np.array_equal(o[k].view(np.uint32), ref), f"{method} {name}: capacity {cap} differs"
 
 
+@pytest.mark.parametrize("method", ["wcsph", "dfsph", "pbf"])
+def test_fixup_gather_without_packed_records(method):
+    """The grid build packs mass, rest density, flags and mask of each particle
+    into one record before its fix-up gathers them in sorted order; a
+    workspace without that buffer (packed = NULL) gathers the four arrays one
+    by one.  Both routes must give bit-identical states."""
+    model, cfg = _compressed_case(method, seed=5)
+    dt = 2.0e-4 if method == "wcsph" else 1.0e-3
+    outs = []
+    for packed in (True, False):
+        solver = make_solver(method, model, cfg)
+        if not packed:
+            solver._work.packed = None
+        s_in, s_out = model.state(), model.state()
+        for _ in range(3):
+            solver.step(s_in, s_out, None, None, dt)
+            s_in, s_out = s_out, s_in
+        torch.cuda.synchronize()
+        outs.append((s_in.particle_q.numpy(), s_in.particle_qd.numpy(), s_in.sph.rho.numpy()))
+    for k, name in enumerate(("x", "v", "rho")):
+        assert np.array_equal(outs[0][k].view(np.uint32), outs[1][k].view(np.uint32)), f"{method} {name}"
+
+
 def test_neighbour_lists_past_2_31_entries():
     """Lists whose entry index (entry * particles + slot) passes 2^31: 1M
     uniform particles plus a ball of 2,400 particles well inside one smoothing
