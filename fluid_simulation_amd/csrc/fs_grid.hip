@@ -7,7 +7,9 @@
 //                     previous sorted order
 //   1. k_hash_count   per particle, in the previous build's sorted order:
 //                     bucket id, rank = atomicAdd(count), both stored at the
-//                     processing position
+//                     processing position; alongside, in original order, the
+//                     mass, rest density, flags and mask of each point packed
+//                     into one record for the fix-up
 //   2. k_bucket_scan  single-pass decoupled look-back exclusive scan of the
 //                     dense bucket counts -> cell_start[dims+1]
 //   3. k_scatter      slot = cell_start[b] + rank for the point at each
@@ -127,6 +129,7 @@ struct HashArgs {
     int* cell_of;
     int* rank;
     int* count;
+    GatherOp pack;      // with pack.packed set, also packs every point's fix-up words
 };
 
 // Position of point i after the solver's pre-op; writes the pre-op outputs.
@@ -202,6 +205,15 @@ __device__ __forceinline__ int hash_point(const HashArgs& a, int i)
     return c;
 }
 
+// The per-point words the fix-up gathers besides the positions and
+// velocities, packed into one 16-byte record of point i.
+__device__ __forceinline__ void pack_point(const GatherOp& ga, int i)
+{
+    const float m = ga.mass ? ga.mass[i] : 0.0f;
+    const float rd = ga.wvol ? ga.rest_density[i] : 0.0f;
+    ga.packed[i] = make_int4(__float_as_int(m), __float_as_int(rd), ga.flags[i], ga.mask ? ga.mask[i] : 0);
+}
+
 // Bucket every point and take a slot rank from its bucket counter.  Lanes are
 // fed in the previous build's sorted order when there is one, so equal buckets
 // sit in runs of adjacent lanes: the first lane of each run adds the run's
@@ -219,6 +231,7 @@ __global__ __launch_bounds__(kElemBlock) void k_hash_count(HashArgs a)
     const int i = valid ? (a.order ? a.order[t] : t) : 0;
     int c = -1;
     if (valid) c = hash_point<MODE>(a, i);
+    if (valid && a.pack.packed) pack_point(a.pack, t);   // original order: coalesced
     const int lane = threadIdx.x & 63;
     const int prev = __shfl_up(c, 1, 64);
     const bool head = valid && (lane == 0 || prev != c);
@@ -386,17 +399,6 @@ __global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __rest
     count[c] = 0;
 }
 
-// The per-point words the fix-up gathers besides the positions and
-// velocities, packed in original order (coalesced) into one record per point.
-__global__ __launch_bounds__(kElemBlock) void k_pack_static(int n, GatherOp ga)
-{
-    const int i = blockIdx.x * kElemBlock + threadIdx.x;
-    if (i >= n) return;
-    const float m = ga.mass ? ga.mass[i] : 0.0f;
-    const float rd = ga.wvol ? ga.rest_density[i] : 0.0f;
-    ga.packed[i] = make_int4(__float_as_int(m), __float_as_int(rd), ga.flags[i], ga.mask ? ga.mask[i] : 0);
-}
-
 __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __restrict__ scatter,
                                                            const int* __restrict__ start, int* __restrict__ point_ids,
                                                            int* __restrict__ point_cells, GatherOp ga)
@@ -521,6 +523,7 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
     // the previous build's sorted order, when it covered the same points
     const int* order = reorder ? g->point_ids : nullptr;
     HashArgs ha{gv, pre, n, order, g->cell_of, g->rank, g->cell_count};
+    if (ga.pos && ga.packed) ha.pack = ga;
     const int nb = blocks_for(n, kElemBlock);
     if (order && nb > 0 && pre.mode != kPrePlain) {
         // pre-op in original order first; the hash then buckets its output
@@ -545,7 +548,6 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
               reinterpret_cast<int2*>(g->scatter_ids),
               g->cell_count, g->scan_state, ntiles);
     if (nb == 0) return FS_OK;
-    if (ga.pos && ga.packed) FS_LAUNCH(k_pack_static, dim3(nb), dim3(kElemBlock), 0, s, n, ga);
     FS_LAUNCH(k_fix_gather, dim3(nb), dim3(kElemBlock), 0, s, n, reinterpret_cast<const int2*>(g->scatter_ids),
               g->cell_start, g->point_ids, g->point_cells, ga);
     return FS_OK;
