@@ -128,11 +128,21 @@ def _grid_points(kind):
     if kind == "faces":
         g = np.arange(-3, 4, dtype=np.float32) * np.float32(0.25)
         return np.ascontiguousarray(np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)), 0.25
+    if kind == "far_clamped":
+        # cell coordinates at and below -2^20 hash to bucket 0 on their axis,
+        # so a query there visits that bucket once per cell
+        h = 0.02
+        edge = -(2 ** 20) * h
+        near = np.column_stack([rng.uniform(edge - 0.1, edge + 0.1, 150), rng.uniform(-0.04, 0.04, (150, 2))])
+        far = np.column_stack([rng.uniform(-2.2e4, -2.1e4, 100), rng.uniform(-0.04, 0.04, (100, 2))])
+        far_y = np.column_stack([rng.uniform(-0.04, 0.04, 60), rng.uniform(edge - 0.1, edge + 0.1, 60),
+                                 rng.uniform(-0.04, 0.04, 60)])
+        return np.vstack([near, far, far_y, [[-1.0e12, 0.0, 0.0]]]).astype(np.float32), h
     raise KeyError(kind)
 
 
 @pytest.mark.parametrize("round_mode", [0, 1])
-@pytest.mark.parametrize("kind", ["negative", "aliasing", "faces"])
+@pytest.mark.parametrize("kind", ["negative", "aliasing", "faces", "far_clamped"])
 def test_grid_build_matches_numpy_restatement(kind, round_mode, oracle):
     pts, h = _grid_points(kind)
     g = oracle.OracleGrid(precision="f32", round_mode=round_mode).build(pts, h)
@@ -158,23 +168,27 @@ def test_floor_and_truncation_differ_only_below_zero(oracle):
     assert f[2] == t[2]
 
 
-@pytest.mark.parametrize("kind", ["negative", "aliasing", "faces"])
+@pytest.mark.parametrize("kind", ["negative", "aliasing", "faces", "far_clamped"])
 def test_candidates_are_bucket_members_in_visit_order(kind, oracle):
     pts, h = _grid_points(kind)
     g = oracle.OracleGrid().build(pts, h)
     off, idx = g.candidates(pts, h)
     cells, _ = _np_cells(pts, h)
+    repeats = 0
     for i in range(0, len(pts), 7):
-        order = {b: k for k, b in enumerate(_np_visited(pts[i], h))}
-        members = [j for j in range(len(pts)) if cells[j] in order]
-        members.sort(key=lambda j: (order[cells[j]], j))
+        visited = _np_visited(pts[i], h)
+        repeats += len(visited) - len(set(visited))
+        # every visit of a bucket lists its members by original index
+        members = [j for b in visited for j in np.nonzero(cells == b)[0]]
         np.testing.assert_array_equal(idx[off[i]:off[i + 1]], members)
+    if kind == "far_clamped":
+        assert repeats > 0
     if kind == "aliasing":   # some candidate lies 128 cells away in x
         far = [abs(pts[j, 0] - pts[i, 0]) > 100 * h for i in range(len(pts)) for j in idx[off[i]:off[i + 1]]]
         assert any(far)
 
 
-@pytest.mark.parametrize("kind", ["negative", "aliasing", "faces"])
+@pytest.mark.parametrize("kind", ["negative", "aliasing", "faces", "far_clamped"])
 def test_within_h_sets_match_brute_force(kind, oracle):
     pts, h = _grid_points(kind)
     g = oracle.OracleGrid().build(pts, h)
