@@ -57,7 +57,8 @@ class FsWcsphParams(ctypes.Structure):
 class FsWcsphIO(ctypes.Structure):
     _fields_ = [
         ("x_in", c_void_p), ("v_in", c_void_p), ("x_out", c_void_p), ("v_out", c_void_p),
-        ("rho", c_void_p), ("pressure", c_void_p), ("accel", c_void_p),
+        ("rho", c_void_p), ("pressure", c_void_p), ("accel", c_void_p), ("rho_in", c_void_p),
+        ("pressure_in", c_void_p),
     ]
 
 
@@ -74,7 +75,7 @@ class FsDfsphParams(ctypes.Structure):
         ("smoothing_length", c_float), ("particle_radius", c_float), ("bound", c_float * 3),
         ("boundary_damping", c_float), ("dt", c_float), ("max_kappa", c_float), ("max_kappa_v", c_float),
         ("max_rho_iterations", c_int32), ("max_vel_iterations", c_int32), ("warm_start", c_int32),
-        ("min_neighbors_for_factor", c_int32),
+        ("min_neighbors_for_factor", c_int32), ("exact_residual", c_int32),
     ]
 
 
@@ -117,7 +118,7 @@ class FsScene(ctypes.Structure):
         ("particle_offset", c_int32), ("particle_count", c_int32), ("bucket_base", c_int32), ("dims", c_int32 * 3),
         ("smoothing_length", c_float), ("cell_width_inv", c_float), ("particle_radius", c_float),
         ("bound", c_float * 3), ("boundary_damping", c_float), ("dt", c_float),
-        ("clamp_negative_pressure", c_int32), ("pad_", c_int32),
+        ("clamp_negative_pressure", c_int32), ("pad_", c_int32), ("cfl_smoothing_length", c_double),
     ]
 
 

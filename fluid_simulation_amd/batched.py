@@ -168,6 +168,7 @@ class BatchedWCSPH:
             t.dims[:] = dims
             h = np.float32(prm.smoothing_length)
             t.smoothing_length = float(h)
+            t.cfl_smoothing_length = float(prm.smoothing_length)
             t.cell_width_inv = float(np.float32(1.0) / h)
             t.particle_radius = float(np.float32(prm.particle_radius))
             t.bound[:] = [float(np.float32(b)) for b in prm.bound]

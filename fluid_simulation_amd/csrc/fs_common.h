@@ -162,7 +162,7 @@ __device__ __forceinline__ v3 spiky_grad(v3 r, float h) { return spiky_grad_r(r,
 __device__ __forceinline__ v3 acc_non_pressure(v3 a, float rho_nei, v3 vel, v3 vel_nei, float mass, float mass_nei,
                                                float mu, float gamma, v3 d, float e_dist, float distance, float h)
 {
-    a = vsub(a, smul(gamma / mass * mass_nei, muls(d, cubic_w(e_dist, h))));
+    a = vsub(a, muls(smul(gamma / mass * mass_nei, d), cubic_w(e_dist, h)));
     float v_current_nei = vdot(vsub(vel, vel_nei), d);
     float s = 10.0f * mu * (mass_nei / fmax_(rho_nei, 1.0e-6f)) * v_current_nei /
               (distance * distance + 0.01f * (h * h));
@@ -174,7 +174,7 @@ __device__ __forceinline__ v3 pbf_acc_non_pressure(v3 a, float rho_nei, v3 vel, 
                                                    float mass_nei, float mu, float gamma, v3 d, float e_dist,
                                                    float distance, float h)
 {
-    if (gamma != 0.0f) a = vsub(a, smul(gamma / mass * mass_nei, muls(d, poly6_w(e_dist, h))));
+    if (gamma != 0.0f) a = vsub(a, muls(smul(gamma / mass * mass_nei, d), poly6_w(e_dist, h)));
     if (mu != 0.0f) {
         float v_current_nei = vdot(vsub(vel, vel_nei), d);
         if (v_current_nei != 0.0f) {

@@ -123,12 +123,13 @@ __global__ void k_scene_dt(fs_scene* scenes, int num_scenes, uint32_t* maxima, d
         if (!isfinite(v2) || !isfinite(a2)) atomicMax(status, 1);
         const double speed = sqrt((double)v2);
         const double accel_max = sqrt((double)a2);
-        const double h = (double)scenes[sidx].smoothing_length;
+        const double h = scenes[sidx].cfl_smoothing_length > 0.0 ? scenes[sidx].cfl_smoothing_length
+                                                                 : (double)scenes[sidx].smoothing_length;
         double limit = frame_dt / substeps;
         const double sound_speed = sqrt(stiffness * exponent / rest_density);
         limit = fmin(limit, 0.25 * h / (sound_speed + speed));
         if (accel_max > 0.0) limit = fmin(limit, 0.25 * sqrt(h / accel_max));
-        if (viscosity > 0.0) limit = fmin(limit, 0.025 * h * h / viscosity);
+        if (viscosity > 0.0) limit = fmin(limit, 0.025 * (h * h) / viscosity);   // 0.025 * h**2 / viscosity
         dt = fmin(rem, limit);
         if (dt < 1.0e-10) atomicMax(status, 2);
         remaining[sidx] = rem - dt;

@@ -141,19 +141,91 @@ __device__ void residual_partials(double v, double c, double* partials)
     }
 }
 
-// residual = mean over dynamic particles of max(x, 0); stop when
-// iter > 0 and residual <= eta  (solver_dfsph.py:579-596, 733-735, 849-851)
-__global__ __launch_bounds__(kPassBlock) void k_dfsph_residual(const double* __restrict__ partials, int nb,
-                                                               int* control, int stop_idx, int it_idx, int iter,
-                                                               double eta)
+// The reference's residual is numpy's float32 mean over the dynamic
+// particles, in original order (solver_dfsph.py:579-596): np.add.reduce walks
+// the compressed array in buffers of 8192 elements, adding each buffer's
+// pairwise sum to a float32 running total, and np.mean divides that total by
+// the count in float64 and rounds the quotient to float32.  The pairwise sum
+// of one buffer splits a range of n > 128 elements at n/2 rounded down to a
+// multiple of 8 and sums a range of 8..128 elements with eight interleaved
+// accumulators, fewer than 8 one after another.
+constexpr int kNpBuffer = 8192;
+constexpr int kNpLeaf = 128;
+
+__device__ __forceinline__ int np_split(int n)
+{
+    int n2 = n / 2;
+    return n2 - n2 % 8;
+}
+
+// pairwise sum of a range of at most kNpLeaf elements
+__device__ float np_leaf_sum(const float* __restrict__ a, int n)
+{
+    if (n < 8) {
+        float r = -0.0f;
+        for (int i = 0; i < n; ++i) r += a[i];
+        return r;
+    }
+    float r0 = a[0], r1 = a[1], r2 = a[2], r3 = a[3], r4 = a[4], r5 = a[5], r6 = a[6], r7 = a[7];
+    int i = 8;
+    for (; i < n - (n % 8); i += 8) {
+        r0 += a[i];
+        r1 += a[i + 1];
+        r2 += a[i + 2];
+        r3 += a[i + 3];
+        r4 += a[i + 4];
+        r5 += a[i + 5];
+        r6 += a[i + 6];
+        r7 += a[i + 7];
+    }
+    float res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+    for (; i < n; ++i) res += a[i];
+    return res;
+}
+
+// The residual value of one particle as the reference forms it:
+// np.maximum(x, 0.0) keeps a NaN.
+__device__ __forceinline__ float np_max0(float x) { return x != x ? x : (x >= 0.0f ? x : 0.0f); }
+
+struct ResidualArgs {
+    const double* __restrict__ partials;   // per-block (sum, count) of the rate pass
+    int nb;
+    int* control;
+    int stop_idx, it_idx, iter;
+    double eta;
+    // the exact form: values in original order (vals - sub, sub optional) of
+    // the dynamic particles, and a scratch of n floats for their compression
+    int n;
+    const int* __restrict__ flags;
+    const int* __restrict__ mask;
+    const float* __restrict__ vals;
+    const float* __restrict__ sub;
+    float* scratch;
+    int force_exact;   // decide every iteration with the exact float32 mean
+};
+
+// One block.  The rate pass's float64 partial sums give the mean to about
+// 1e-15; numpy's float32 mean differs from the exact mean by less than
+// (40 + buffers) * 2^-24 relative (each value passes through at most 25
+// roundings in its leaf, 7 pairwise levels and one per later buffer; all
+// values are >= 0).  A residual that far from eta decides the break as the
+// float32 mean would; a closer one (or any NaN) is decided by forming the
+// float32 mean exactly as numpy does.
+__global__ __launch_bounds__(kPassBlock) void k_dfsph_residual(ResidualArgs a)
 {
     __shared__ double s_v[kPassBlock / 64];
     __shared__ double s_c[kPassBlock / 64];
-    if (control[stop_idx]) return;
+    __shared__ int s_exact;
+    __shared__ int s_scan[kPassBlock / 64 + 1];
+    __shared__ int s_leaf_lo[2 * kNpBuffer / kNpLeaf + 8];
+    __shared__ int s_leaf_n[2 * kNpBuffer / kNpLeaf + 8];
+    __shared__ float s_leaf_sum[2 * kNpBuffer / kNpLeaf + 8];
+    __shared__ int s_nleaf;
+    if (a.control[a.stop_idx]) return;
     double tv = 0.0, tc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kPassBlock) {
-        tv += partials[2 * b];
-        tc += partials[2 * b + 1];
+    for (int b = threadIdx.x; b < a.nb; b += kPassBlock) {
+        tv += a.partials[2 * b];
+        tc += a.partials[2 * b + 1];
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -172,9 +244,139 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_residual(const double* __r
             sv += s_v[w];
             sc += s_c[w];
         }
-        const double residual = sc > 0.0 ? sv / sc : 0.0;
-        control[it_idx] += 1;
-        if (iter > 0 && residual <= eta) control[stop_idx] = 1;
+        a.control[a.it_idx] += 1;
+        int exact = 0;
+        if (a.iter > 0) {
+            if (!(sc > 0.0)) {
+                // no dynamic particle: the reference's residual is 0.0
+                if (0.0 <= a.eta) a.control[a.stop_idx] = 1;
+            } else {
+                const double residual = sv / sc;
+                const double buffers = ceil(sc / (double)kNpBuffer);
+                const double margin = residual * ((40.0 + buffers) * 0x1.0p-24 + sc * 0x1.0p-52);
+                if (a.force_exact || !(residual == residual) || fabs(residual - a.eta) <= margin) exact = 1;
+                else if (residual <= a.eta) a.control[a.stop_idx] = 1;
+            }
+        }
+        s_exact = exact;
+    }
+    __syncthreads();
+    if (!s_exact) return;
+
+    // compress the dynamic particles' values into scratch, in original order
+    constexpr int kItems = 8;
+    int base_out = 0;
+    for (int base = 0; base < a.n; base += kPassBlock * kItems) {
+        const int i0 = base + threadIdx.x * kItems;
+        float v[kItems];
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            const int i = i0 + u;
+            const bool dyn = i < a.n && (a.flags[i] & 1) != 0 && a.mask[i] == 0;
+            v[u] = 0.0f;
+            if (dyn) {
+                const float x = a.sub ? a.vals[i] - a.sub[i] : a.vals[i];
+                v[u] = np_max0(x);
+                ++cnt;
+            }
+        }
+        // exclusive scan of the per-thread counts over the block
+        int incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        if (lane == 63) s_scan[wid] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int run = 0;
+            for (int w = 0; w < kPassBlock / 64; ++w) {
+                const int t = s_scan[w];
+                s_scan[w] = run;
+                run += t;
+            }
+            s_scan[kPassBlock / 64] = run;
+        }
+        __syncthreads();
+        int out = base_out + s_scan[wid] + incl - cnt;
+#pragma unroll
+        for (int u = 0; u < kItems; ++u) {
+            const int i = i0 + u;
+            if (i < a.n && (a.flags[i] & 1) != 0 && a.mask[i] == 0) a.scratch[out++] = v[u];
+        }
+        base_out += s_scan[kPassBlock / 64];
+        __syncthreads();
+    }
+    const int nd = base_out;
+
+    // buffer by buffer: pairwise sums of the leaves in parallel, the tree and
+    // the running total on one lane
+    float total = 0.0f;
+    for (int b0 = 0; b0 < nd; b0 += kNpBuffer) {
+        const int m = min(kNpBuffer, nd - b0);
+        if (threadIdx.x == 0) {
+            int st_lo[32], st_n[32], sp = 0, nl = 0;
+            st_lo[sp] = 0;
+            st_n[sp++] = m;
+            while (sp > 0) {
+                --sp;
+                const int lo = st_lo[sp], len = st_n[sp];
+                if (len <= kNpLeaf) {
+                    s_leaf_lo[nl] = lo;
+                    s_leaf_n[nl++] = len;
+                } else {
+                    const int n2 = np_split(len);
+                    st_lo[sp] = lo + n2;
+                    st_n[sp++] = len - n2;
+                    st_lo[sp] = lo;
+                    st_n[sp++] = n2;
+                }
+            }
+            s_nleaf = nl;
+        }
+        __syncthreads();
+        for (int l = threadIdx.x; l < s_nleaf; l += kPassBlock)
+            s_leaf_sum[l] = np_leaf_sum(a.scratch + b0 + s_leaf_lo[l], s_leaf_n[l]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // post-order walk of the same tree: a frame is (length, stage)
+            int f_n[32], f_stage[32], fp = 0, next = 0, vp = 0;
+            float vals[32];
+            f_n[fp] = m;
+            f_stage[fp++] = 0;
+            while (fp > 0) {
+                const int len = f_n[fp - 1];
+                if (len <= kNpLeaf) {
+                    --fp;
+                    vals[vp++] = s_leaf_sum[next++];
+                    continue;
+                }
+                const int n2 = np_split(len);
+                if (f_stage[fp - 1] == 0) {
+                    f_stage[fp - 1] = 1;
+                    f_n[fp] = n2;
+                    f_stage[fp++] = 0;
+                } else if (f_stage[fp - 1] == 1) {
+                    f_stage[fp - 1] = 2;
+                    f_n[fp] = len - n2;
+                    f_stage[fp++] = 0;
+                } else {
+                    --fp;
+                    const float rgt = vals[--vp];
+                    const float lft = vals[--vp];
+                    vals[vp++] = lft + rgt;
+                }
+            }
+            total = total + vals[0];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        // np.mean: float32(float64(total) / count); the break test in float64
+        const float mean = nd > 0 ? (float)((double)total / (double)nd) : 0.0f;
+        if ((double)mean <= a.eta) a.control[a.stop_idx] = 1;
     }
 }
 
@@ -192,6 +394,8 @@ struct WcsphDensityArgs {
     const float* __restrict__ exponent;
     int clamp;
     const float* dt_device;   // plain pass: a device dt of 0 makes the pass idle
+    const float* rho_in;      // idle pass: carried over to rho / pressure (optional)
+    const float* pressure_in;
     float* rho;
     float* pressure;
     float4* sca;    // rho, p, p/max(rho^2,1e-6), -
@@ -267,7 +471,11 @@ __global__ __launch_bounds__(BLOCK) void k_wcsph_density(WcsphDensityArgs a)
             }
         });
     } else {
-        if (a.dt_device && *a.dt_device == 0.0f) return;
+        if (a.dt_device && *a.dt_device == 0.0f) {   // idle substep: carry rho and p over
+            if (a.rho_in) a.rho[s] = a.rho_in[s];
+            if (a.pressure_in) a.pressure[s] = a.pressure_in[s];
+            return;
+        }
         wcsph_density_lane<BLOCK, CAP>(a, a.c, a.clamp, s, lds);
     }
 }
@@ -332,7 +540,7 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
             const float mass_nei = xj.w;
             const v3 grad = cubic_grad_r(d, distance, h);
             if (gamma != 0.0f)
-                acc = vsub(acc, smul(gamma / m_i * mass_nei, muls(d, cubic_w(fmax_(distance, pr), h))));
+                acc = vsub(acc, muls(smul(gamma / m_i * mass_nei, d), cubic_w(fmax_(distance, pr), h)));
             if (mu != 0.0f) {
                 const float4 vj = nb.v;
                 float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
@@ -650,13 +858,21 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
     // count = active candidates other than the particle itself, no distance
     // test (solver_dfsph.py:150-160)
     int count = 0;
-    if (*a.any_inactive) {
+    // A query range repeats a bucket only where an axis has two or more cells
+    // at or below -kOrigin (they all hash to bucket 0 of that axis), i.e.
+    // where its first cell lies below -kOrigin; the particle's own bucket may
+    // then be visited more than once, and the reference skips it every time.
+    const GridView& gv = a.c.g;
+    const bool repeats = cell_coord((xi.x - h) * gv.inv, gv.round_mode) < -kOrigin ||
+                         cell_coord((xi.y - h) * gv.inv, gv.round_mode) < -kOrigin ||
+                         cell_coord((xi.z - h) * gv.inv, gv.round_mode) < -kOrigin;
+    if (*a.any_inactive || repeats) {
         auto counter = [&](int k) { count += (k != s && (a.c.sflags[k] & kActive)) ? 1 : 0; };
         SCAN_NEIGHBORS_B(a.c, s, xi.x, xi.y, xi.z, counter, body);
     } else {
-        // every particle active: all candidates but the particle itself, which
-        // is always among its own candidates exactly once (its bucket lies in
-        // the query range and a range never repeats a bucket)
+        // every particle active and no bucket repeated: all candidates but
+        // the particle itself, which is among its own candidates exactly once
+        // (its bucket lies in the query range)
         const int total = SCAN_NEIGHBORS_B(a.c, s, xi.x, xi.y, xi.z, NoCount{}, body);
         count = total > 0 ? total - 1 : 0;
     }
@@ -719,7 +935,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureA
                 const v3 d = vsub(xi, xyz(xj));
                 const float dist = vlen(d);
                 if (gamma != 0.0f)
-                    acc = vsub(acc, smul(gamma / m_i * xj.w, muls(d, cubic_w(fmax_(dist, pr), h))));
+                    acc = vsub(acc, muls(smul(gamma / m_i * xj.w, d), cubic_w(fmax_(dist, pr), h)));
                 if (mu != 0.0f) {
                     const float4 vj = nb.v;
                     float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
@@ -791,16 +1007,17 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
             if (a.density_loop) {
                 const float rs = si.x + a.dt * div_val;
                 a.rho_star[i] = rs;
-                const float err = fmax_(rs - a.rest_density[i], 0.0f);
+                const float diff = rs - a.rest_density[i];
+                const float err = fmax_(diff, 0.0f);
                 if (dyn) {
-                    res_v = (double)err;
+                    res_v = (double)np_max0(diff);
                     res_c = 1.0;
                     kap = fmin_((err / (a.dt * a.dt)) * si.y, a.max_kappa);
                 }
             } else {
                 const float dc = fmax_(div_val, 0.0f);
                 if (dyn) {
-                    res_v = (double)dc;
+                    res_v = (double)np_max0(div_val);
                     res_c = 1.0;
                     kap = fmin_((1.0f / a.dt) * dc * si.x * si.y, a.max_kappa);
                 }
@@ -1042,7 +1259,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_nonpressure(PbfNonPressureAr
                         const v3 d = vsub(xi, xyz(xj));
                         const float dist = vlen(d);
                         if (gamma != 0.0f)
-                            acc = vsub(acc, smul(gamma / m_i * xj.w, muls(d, poly6_w(fmax_(dist, pr), h))));
+                            acc = vsub(acc, muls(smul(gamma / m_i * xj.w, d), poly6_w(fmax_(dist, pr), h)));
                         if (mu != 0.0f) {
                             const float4 vj = nb.v;
                             float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
@@ -1501,6 +1718,8 @@ static int32_t wcsph_density_eos(fs_grid* g, fs_work* w, const fs_model* m, cons
     da.exponent = m->exponent;
     da.clamp = prm->clamp_negative_pressure;
     da.dt_device = prm->dt_device;
+    da.rho_in = io->rho_in;
+    da.pressure_in = io->pressure_in;
     da.rho = io->rho;
     da.pressure = io->pressure;
     da.sca = reinterpret_cast<float4*>(w->sca);
@@ -1780,6 +1999,8 @@ int32_t fs_batch_wcsph_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_
     da.exponent = m->exponent;
     da.clamp = 1;
     da.dt_device = nullptr;
+    da.rho_in = nullptr;
+    da.pressure_in = nullptr;
     da.rho = io->rho;
     da.pressure = io->pressure;
     da.sca = reinterpret_cast<float4*>(w->sca);
@@ -1962,9 +2183,22 @@ static int32_t df_rate_kappa(const DfPlan& p, int loop)
 // (solver_dfsph.py:574-596, 733-735, 849-851), decided on the device
 static int32_t df_residual(const DfPlan& p, int loop, int iteration)
 {
-    FS_LAUNCH(k_dfsph_residual, dim3(1), dim3(kPassBlock), 0, p.s, p.w->partials, p.nb, p.w->control,
-              loop == 0 ? kStopRho : kStopVel, loop == 0 ? kItRho : kItVel, iteration,
-              loop == 0 ? p.prm->eta_density : p.prm->eta_divergence);
+    ResidualArgs ra;
+    ra.partials = p.w->partials;
+    ra.nb = p.nb;
+    ra.control = p.w->control;
+    ra.stop_idx = loop == 0 ? kStopRho : kStopVel;
+    ra.it_idx = loop == 0 ? kItRho : kItVel;
+    ra.iter = iteration;
+    ra.eta = loop == 0 ? p.prm->eta_density : p.prm->eta_divergence;
+    ra.n = p.n;
+    ra.flags = p.m->flags;
+    ra.mask = p.m->mask;
+    ra.vals = loop == 0 ? p.io->rho_star : p.io->d_rho_dt;
+    ra.sub = loop == 0 ? p.m->rest_density : nullptr;
+    ra.scratch = p.w->x_pred;   // unused by DFSPH otherwise; holds >= n floats
+    ra.force_exact = p.prm->exact_residual;
+    FS_LAUNCH(k_dfsph_residual, dim3(1), dim3(kPassBlock), 0, p.s, ra);
     return FS_OK;
 }
 

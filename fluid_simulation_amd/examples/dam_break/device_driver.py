@@ -68,6 +68,7 @@ class DeviceResidentDamBreak:
         t.particle_offset = 0
         t.particle_count = self.scene.count
         t.smoothing_length = float(np.float32(sol.smoothing_length))
+        t.cfl_smoothing_length = float(sol.smoothing_length)   # the host policy's float64 h
         t.dt = 0.0
         self._table = torch.frombuffer(bytearray(bytes(t)), dtype=torch.uint8).to(self.device)
         off = _lib.FsScene.dt.offset

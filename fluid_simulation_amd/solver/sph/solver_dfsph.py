@@ -4,10 +4,13 @@ Drop-in for the reference's ``solver/sph/solver_dfsph.py`` ``SolverDFSPH``:
 same ``SphConfig`` fields and defaults (:446-463), ``sph:`` attributes
 (:465-535) and ``step`` (:598-898).  The constant-density and divergence-free
 loops keep the reference's early exit (``iter > 0 and residual <= eta``,
-:733-735, :849-851) but decide it on the device -- the residual (mean over
-dynamic particles of max(x, 0)) is reduced in float64 inside the rate pass --
-so ``step`` never synchronises the host.  ``last_iterations()`` reads back the
-loop counts of the previous step.
+:733-735, :849-851) but decide it on the device, so ``step`` never
+synchronises the host.  The residual is the reference's numpy float32 mean
+over the dynamic particles (:579-596): a float64 mean from the rate pass
+decides whenever it lies farther from eta than float32 rounding reaches, and
+the float32 mean is formed in numpy's summation order otherwise (or always,
+with ``exact_residual = True``).  ``last_iterations()`` reads back the loop
+counts of the previous step.
 """
 from __future__ import annotations
 
@@ -57,6 +60,10 @@ class SolverDFSPH(ParticleSolver):
         register(builder, "neighbor_count", int32, 0, AttributeAssignment.STATE)
         register(builder, "v_star", vec3, (0.0, 0.0, 0.0), AttributeAssignment.STATE)
 
+    #: form numpy's float32 residual for every break test, not only for
+    #: residuals within float32 rounding of eta (same decisions, slower)
+    exact_residual = False
+
     def __init__(self, model, config: SphConfig, **kwargs):
         super().__init__(model, config, **kwargs)
         self._iterations = torch.zeros(4, dtype=torch.int32, device=model.device)
@@ -86,6 +93,7 @@ class SolverDFSPH(ParticleSolver):
         p.max_vel_iterations = int(cfg.max_vel_iterations)
         p.warm_start = 1 if cfg.warm_start else 0
         p.min_neighbors_for_factor = int(cfg.min_neighbors_for_factor)
+        p.exact_residual = 1 if self.exact_residual else 0
         si, so = state_in.sph, state_out.sph
         io = _lib.FsDfsphIO()
         io.x_in = ptr(state_in.particle_q)

@@ -75,6 +75,8 @@ class SolverWCSPH(ParticleSolver):
         io.rho = ptr(state_out.sph.rho)
         io.pressure = ptr(state_out.sph.pressure)
         io.accel = ptr(self._particle_accel)
+        io.rho_in = ptr(state_in.sph.rho)
+        io.pressure_in = ptr(state_in.sph.pressure)
         m = self._model_struct(with_eos=True)
         lib = _lib.load()
         _lib.check(lib.fs_wcsph_step(ctypes.byref(self.model.particle_grid._c), ctypes.byref(self._work),

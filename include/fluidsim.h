@@ -213,7 +213,9 @@ typedef struct fs_wcsph_params {
     /* Optional device scalar that replaces `dt` (NULL: use `dt`).  With it a
      * substep needs no host value, so a device-side CFL (fs_batch_cfl_dt on
      * a one-scene table) and graph capture drive whole frames.  A device dt
-     * of 0 leaves the state, rho, pressure and acceleration unchanged. */
+     * of 0 copies the state (x, v, and rho/pressure when io->rho_in and
+     * io->pressure_in are set) to the outputs and leaves the acceleration
+     * unchanged. */
     const float* dt_device;
 } fs_wcsph_params;
 
@@ -225,6 +227,11 @@ typedef struct fs_wcsph_io {
     float* rho;          /* state_out.sph.rho     */
     float* pressure;     /* state_out.sph.pressure */
     float* accel;        /* solver._particle_accel [n*3] */
+    /* Optional (may be NULL): state_in.sph.rho / .pressure.  A substep whose
+     * device dt is 0 copies them to rho / pressure, so that an idle substep
+     * carries the whole state over, density and pressure included. */
+    const float* rho_in;
+    const float* pressure_in;
 } fs_wcsph_io;
 
 /* One WCSPH substep: build, compute_density_pressure, acceleration_wcsph,
@@ -289,6 +296,10 @@ typedef struct fs_dfsph_params {
     int32_t max_vel_iterations;
     int32_t warm_start;
     int32_t min_neighbors_for_factor;
+    int32_t exact_residual;    /* 1: form numpy's float32 mean for every break
+                                  test (slow; for checks).  0: only when the
+                                  float64 mean lies within float32 rounding
+                                  of eta                                      */
 } fs_dfsph_params;
 
 typedef struct fs_dfsph_io {
@@ -313,9 +324,13 @@ typedef struct fs_dfsph_io {
 } fs_dfsph_io;
 
 /* One DFSPH substep (SolverDFSPH.step, solver_dfsph.py:598-898).  The
- * residual early exits (:733-735, :849-851) are decided on the device:
- * residual = mean over dynamic particles of max(x, 0), accumulated in
- * float64; the remaining passes of a finished loop return at once. */
+ * residual early exits (:733-735, :849-851) are decided on the device with
+ * the reference's arithmetic: residual = numpy's float32 mean of max(x, 0)
+ * over the dynamic particles in original order (:579-596).  A float64 mean
+ * decides when it lies farther from eta than float32 rounding can move the
+ * float32 mean; otherwise the float32 mean is formed in numpy's order
+ * (8192-element buffers, pairwise within each).  The remaining passes of a
+ * finished loop return at once. */
 int32_t fs_dfsph_step(fs_grid* grid, fs_work* work, const fs_model* model, const fs_dfsph_params* params,
                       const fs_dfsph_io* io, fs_stream_t stream);
 /* The passes of fs_dfsph_step, one call each; the substep is exactly:
@@ -464,6 +479,9 @@ typedef struct fs_scene {
                                      acceleration as they are               */
     int32_t clamp_negative_pressure;
     int32_t pad_;
+    double cfl_smoothing_length;  /* h as the host holds it (float64), for the
+                                     CFL limits of fs_batch_cfl_dt; 0: use
+                                     smoothing_length                        */
 } fs_scene;
 
 typedef struct fs_batch_io {
@@ -492,7 +510,9 @@ int32_t fs_batch_wcsph_step(fs_grid* grid, fs_work* work, const fs_model* model,
  *   c0 = sqrt(stiffness*exponent/rest_density),
  *   scenes[s].dt = min(remaining[s], limit) while remaining[s] > 1e-12
  *   (else 0), remaining[s] -= dt,
- * with vmax/amax the scene's maxima of |v| / |a| (a ignored when use_accel is
+ * with h the scene's cfl_smoothing_length (float64, as the host's
+ * SolverWCSPH.smoothing_length) and vmax/amax the scene's maxima of |v| / |a|
+ * (a ignored when use_accel is
  * 0).  status[0] = 1 if a maximum is non-finite, 2 if a scene's dt falls
  * below 1e-10 (the reference's RuntimeError cases).  scratch: device uint32
  * [2*num_scenes], zero on first use and left zero.  max_scene_particles (the

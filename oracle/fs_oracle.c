@@ -300,7 +300,7 @@ static inline v3 spiky_grad(v3 r, R smoothing_length)
 static inline v3 acc_non_pressure(v3 a, R rho_nei, v3 vel, v3 vel_nei, R mass, R mass_nei, R mu, R gamma,
                                   v3 d, R e_dist, R distance, R h)
 {
-    a = vsub(a, smul(gamma / mass * mass_nei, muls(d, cubic_w(e_dist, h))));
+    a = vsub(a, muls(smul(gamma / mass * mass_nei, d), cubic_w(e_dist, h)));
     R v_current_nei = vdot(vsub(vel, vel_nei), d);
     R s = RC(10.0) * mu * (mass_nei / rmax(rho_nei, RC(1.0e-6))) * v_current_nei
           / (distance * distance + RC(0.01) * (h * h));
@@ -312,7 +312,7 @@ static inline v3 acc_non_pressure(v3 a, R rho_nei, v3 vel, v3 vel_nei, R mass, R
 static inline v3 pbf_acc_non_pressure(v3 a, R rho_nei, v3 vel, v3 vel_nei, R mass, R mass_nei, R mu, R gamma,
                                       v3 d, R e_dist, R distance, R h)
 {
-    if (gamma != RC(0.0)) a = vsub(a, smul(gamma / mass * mass_nei, muls(d, poly6_w(e_dist, h))));
+    if (gamma != RC(0.0)) a = vsub(a, muls(smul(gamma / mass * mass_nei, d), poly6_w(e_dist, h)));
     if (mu != RC(0.0)) {
         R v_current_nei = vdot(vsub(vel, vel_nei), d);
         if (v_current_nei != RC(0.0)) {

@@ -425,7 +425,8 @@ def dfsph_step(model: OModel, state_in: OState, cfg: DfsphConfig, dt: float, pre
     sph["v_star"] = np.zeros((n, 3), f)
     out = OState(np.zeros_like(x_in), np.zeros_like(v_in), sph)
     accel = np.zeros((n, 3), f)
-    extras = {"accel": accel, "rho_iterations": 0, "vel_iterations": 0}
+    extras = {"accel": accel, "rho_iterations": 0, "vel_iterations": 0, "rho_residuals": [],
+              "vel_residuals": []}
     if n == 0:
         return out, extras
     h = smoothing_length(cfg)
@@ -455,6 +456,7 @@ def dfsph_step(model: OModel, state_in: OState, cfg: DfsphConfig, dt: float, pre
                                            c.r(dt), _p(sph["rho_star"]))
         extras["rho_iterations"] = density_iter + 1
         residual = _dfsph_residual(sph["rho_star"] - m.rest_density, dynamic_mask)
+        extras["rho_residuals"].append(residual)
         if density_iter > 0 and residual <= cfg.eta_density:
             break
         lib.ora_dfsph_kappa_density(ctypes.c_int(n), _p(m.flags), _p(m.mask), _p(sph["rho_star"]),
@@ -482,6 +484,7 @@ def dfsph_step(model: OModel, state_in: OState, cfg: DfsphConfig, dt: float, pre
                                           _p(sph["d_rho_dt"]))
         extras["vel_iterations"] = divergence_iter + 1
         residual = _dfsph_residual(sph["d_rho_dt"], dynamic_mask)
+        extras["vel_residuals"].append(residual)
         if divergence_iter > 0 and residual <= cfg.eta_divergence:
             break
         lib.ora_dfsph_kappa_divergence(ctypes.c_int(n), _p(m.flags), _p(m.mask), _p(sph["d_rho_dt"]),
