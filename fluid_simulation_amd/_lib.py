@@ -14,6 +14,10 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libfluidsim_hip.so")
 
 FS_OK = 0
+FS_ERR_ARG = -1
+FS_ERR_CAPACITY = -2
+FS_ERR_HIP = -3
+FS_ERR_WORKSPACE = -4
 
 
 class FsGrid(ctypes.Structure):
@@ -198,6 +202,9 @@ SIGNATURES = {
                                       POINTER(FsBatchIO), c_void_p]),
     "fs_batch_cfl_dt": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_double, c_double,
                                   c_double, c_double, c_double, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fs_batch_cfl_dt_noalloc": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_double,
+                                          c_double, c_double, c_double, c_double, c_int32, c_void_p, c_void_p,
+                                          c_void_p]),
     "fs_norm_maxima": (c_int32, [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),
     "fs_check_finite": (c_int32, [c_void_p, c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p]),
 }

@@ -3,6 +3,15 @@
 #include "fs_internal.h"
 
 #include <algorithm>
+#include <mutex>
+
+// Scratch of fs_batch_cfl_dt_noalloc: the per-scene maxima, (|v|^2, |a|^2) as
+// float bits, for at most kNoAllocMaxScenes scenes.  Each device holds its own
+// copy of this array until the process ends.  Host code never passes the
+// name itself to a kernel: it takes the device address from
+// hipGetSymbolAddress, once per device (noalloc_scratch below).
+constexpr int kNoAllocMaxScenes = 4096;
+static __device__ __align__(256) uint32_t scratch[2 * kNoAllocMaxScenes];
 
 namespace fs {
 
@@ -19,6 +28,13 @@ __global__ void k_fill_zero(uint32_t* p, int n)
 {
     int i = threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+// zero n words (any n), grid-stride; a kernel rather than hipMemsetAsync so
+// that the clear also records correctly into a captured graph
+__global__ __launch_bounds__(256) void k_fill_zero_n(uint32_t* p, int n)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0u;
 }
 
 // particle_norm_maxima  simulation.py:19-36.  Values are >= 0 (or +inf), so
@@ -137,6 +153,67 @@ __global__ void k_scene_dt(fs_scene* scenes, int num_scenes, uint32_t* maxima, d
     scenes[sidx].dt = (float)dt;
 }
 
+// Device address of `scratch` on the current device, looked up once per device.
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mutex;
+static uint32_t* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+static int32_t noalloc_scratch(hipStream_t stream, uint32_t** out)
+{
+    int dev = 0;
+    int32_t rc = hip_check(hipGetDevice(&dev), "fs_batch_cfl_dt_noalloc: hipGetDevice");
+    if (rc) return rc;
+    if (stream) {
+        int sdev = 0;
+        rc = hip_check(hipStreamGetDevice(stream, &sdev), "fs_batch_cfl_dt_noalloc: hipStreamGetDevice");
+        if (rc) return rc;
+        if (sdev != dev) {
+            set_error("fs_batch_cfl_dt_noalloc: the stream belongs to device %d, the current device is %d", sdev,
+                      dev);
+            return FS_ERR_ARG;
+        }
+    }
+    if (dev < 0 || dev >= kNoAllocMaxDevices) {
+        set_error("fs_batch_cfl_dt_noalloc: device %d outside the supported 0..%d", dev, kNoAllocMaxDevices - 1);
+        return FS_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "fs_batch_cfl_dt_noalloc: hipGetSymbolAddress");
+        if (rc) return rc;
+        g_noalloc_ptr[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return FS_OK;
+}
+
+static int32_t check_cfl_args(const char* who, const fs_scene* scenes, int32_t num_scenes, const float* v,
+                              const float* a, int32_t use_accel, const double* remaining, const int32_t* status,
+                              int32_t substeps)
+{
+    if (!scenes || num_scenes <= 0 || num_scenes > 65535 || !v || (use_accel && !a) || !remaining || !status ||
+        substeps <= 0) {
+        set_error("%s: invalid arguments", who);
+        return FS_ERR_ARG;
+    }
+    return FS_OK;
+}
+
+static int32_t launch_cfl(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_particles, const float* v,
+                          const float* a, int32_t use_accel, double stiffness, double exponent, double rest_density,
+                          double viscosity, double frame_dt, int32_t substeps, double* remaining, uint32_t* maxima,
+                          int32_t* status, hipStream_t s)
+{
+    // blocks per scene: enough for the largest scene (about 8 particles per
+    // lane), at most 128
+    const int per_scene = max_scene_particles > 0 ? std::min(128, blocks_for(max_scene_particles, kRedBlock * 8)) : 8;
+    FS_LAUNCH(k_scene_maxima, dim3(per_scene, num_scenes), dim3(kRedBlock), 0, s, scenes, v, a, use_accel, maxima);
+    FS_LAUNCH(k_scene_dt, dim3(blocks_for(num_scenes, 64)), dim3(64), 0, s, scenes, num_scenes, maxima, stiffness,
+              exponent, rest_density, viscosity, frame_dt, substeps, remaining, status);
+    return FS_OK;
+}
+
 static int red_blocks(int n)
 {
     int b = blocks_for(n, kRedBlock * 8);
@@ -170,19 +247,38 @@ int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_
                         double viscosity, double frame_dt, int32_t substeps, double* remaining, uint32_t* scratch,
                         int32_t* status, fs_stream_t stream)
 {
-    if (!scenes || num_scenes <= 0 || num_scenes > 65535 || !v || (use_accel && !a) || !remaining || !scratch ||
-        !status || substeps <= 0) {
-        set_error("fs_batch_cfl_dt: invalid arguments");
+    int32_t rc = check_cfl_args("fs_batch_cfl_dt", scenes, num_scenes, v, a, use_accel, remaining, status, substeps);
+    if (rc) return rc;
+    if (!scratch) {
+        set_error("fs_batch_cfl_dt: invalid arguments (null scratch)");
         return FS_ERR_ARG;
     }
+    return launch_cfl(scenes, num_scenes, max_scene_particles, v, a, use_accel, stiffness, exponent, rest_density,
+                      viscosity, frame_dt, substeps, remaining, scratch, status, (hipStream_t)stream);
+}
+
+int32_t fs_batch_cfl_dt_noalloc(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_particles, const float* v,
+                                const float* a, int32_t use_accel, double stiffness, double exponent,
+                                double rest_density, double viscosity, double frame_dt, int32_t substeps,
+                                double* remaining, int32_t* status, fs_stream_t stream)
+{
+    int32_t rc = check_cfl_args("fs_batch_cfl_dt_noalloc", scenes, num_scenes, v, a, use_accel, remaining, status,
+                                substeps);
+    if (rc) return rc;
+    if (num_scenes > kNoAllocMaxScenes) {
+        set_error("fs_batch_cfl_dt_noalloc: %d scenes exceed the fixed scratch of %d scenes (use fs_batch_cfl_dt)",
+                  num_scenes, kNoAllocMaxScenes);
+        return FS_ERR_CAPACITY;
+    }
     hipStream_t s = (hipStream_t)stream;
-    // blocks per scene: enough for the largest scene (about 8 particles per
-    // lane), at most 128
-    const int per_scene = max_scene_particles > 0 ? std::min(128, blocks_for(max_scene_particles, kRedBlock * 8)) : 8;
-    FS_LAUNCH(k_scene_maxima, dim3(per_scene, num_scenes), dim3(kRedBlock), 0, s, scenes, v, a, use_accel, scratch);
-    FS_LAUNCH(k_scene_dt, dim3(blocks_for(num_scenes, 64)), dim3(64), 0, s, scenes, num_scenes, scratch, stiffness,
-              exponent, rest_density, viscosity, frame_dt, substeps, remaining, status);
-    return FS_OK;
+    uint32_t* maxima = nullptr;
+    rc = noalloc_scratch(s, &maxima);
+    if (rc) return rc;
+    // the array keeps the previous call's contents: the maxima must start at 0
+    const int words = 2 * num_scenes;
+    FS_LAUNCH(k_fill_zero_n, dim3(std::min(32, blocks_for(words, 256))), dim3(256), 0, s, maxima, words);
+    return launch_cfl(scenes, num_scenes, max_scene_particles, v, a, use_accel, stiffness, exponent, rest_density,
+                      viscosity, frame_dt, substeps, remaining, maxima, status, s);
 }
 
 int32_t fs_check_finite(const float* x, const float* v, int32_t n, const float* density, int32_t nd,

@@ -523,6 +523,20 @@ int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_
                         double viscosity, double frame_dt, int32_t substeps, double* remaining, uint32_t* scratch,
                         int32_t* status, fs_stream_t stream);
 
+/* fs_batch_cfl_dt without the scratch argument; it allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Its scratch is a fixed device array inside the library, one
+ * per device, for at most 4096 scenes (32 KiB); more scenes return
+ * FS_ERR_CAPACITY.  The array is cleared by a kernel at the start of each
+ * call.  Calls that overlap on one device (concurrent streams, or several
+ * host threads) share that array and race: serialise them.  The stream must
+ * belong to the current device. */
+int32_t fs_batch_cfl_dt_noalloc(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_particles, const float* v,
+                                const float* a, int32_t use_accel, double stiffness, double exponent,
+                                double rest_density, double viscosity, double frame_dt, int32_t substeps,
+                                double* remaining, int32_t* status, fs_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Driver diagnostics  (examples/dam_break/simulation.py:19-53)           */
 /* ------------------------------------------------------------------------ */
