@@ -46,6 +46,14 @@ def _lib(precision: str):
     path = os.path.join(HERE, f"libfs_oracle_{precision}.so")
     if not os.path.exists(path):
         build()
+    lib = load_library(path, precision)
+    _LIBS[precision] = lib
+    return lib
+
+
+def load_library(path: str, precision: str):
+    """Load an oracle build (``fs_oracle.c`` compiled with ORACLE_REAL float
+    for "f32", double for "f64") and declare its scalar signatures."""
     lib = ctypes.CDLL(path)
     real = ctypes.c_float if precision == "f32" else ctypes.c_double
     lib._real = real
@@ -66,7 +74,6 @@ def _lib(precision: str):
     lib.ora_poly6_w.argtypes = [real, real]
     lib.ora_grid_candidates.restype = ctypes.c_longlong
     assert lib.ora_real_bytes() == ctypes.sizeof(real)
-    _LIBS[precision] = lib
     return lib
 
 

@@ -136,6 +136,48 @@ def test_cohesion_one_substep(method, oracle):
     assert not np.array_equal(extras0["accel"], extras["accel"])
 
 
+def test_cohesion_pairs_bitwise(oracle):
+    """Isolated pairs, WCSPH without pressure or viscosity: each
+    acceleration is one cohesion term, 0 - ((gamma/m_i*m_j)*d)*W.  The GPU's
+    float32 result must equal the oracle's bit for bit; the other
+    association, (gamma/m_i*m_j)*(d*W), gives other bits for some pairs."""
+    d = 0.01
+    h = 2 * d
+    rng = np.random.default_rng(21)
+    npairs = 400
+    centres = np.stack(np.meshgrid(*(np.arange(8) * 3 * h,) * 3, indexing="ij"), -1).reshape(-1, 3)[:npairs]
+    centres = centres + np.array([0.0, 0.0, 0.3])
+    dirs = rng.normal(size=(npairs, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    r = rng.uniform(0.2, 0.95, npairs) * h
+    pts = np.concatenate([centres, centres + dirs * r[:, None]]).astype(np.float32)
+    n = len(pts)
+    mat = {"rest_density": 1000.0, "surface_tension": 0.0123, "viscosity": 0.0, "stiffness": 0.0,
+           "exponent": 7.0}
+    from fluid_simulation_amd.solver.sph.solver_wcsph import SolverWCSPH
+    cfg = SolverWCSPH.SphConfig(particle_radius=0.5 * d, particle_length=d, smoothing_length_coff=2.0,
+                                boundary_damping=0.0, bound_width=5.0, bound_height=5.0, bound_length=5.0)
+    model = build_model("wcsph", pts, np.zeros_like(pts), mass=1000.0 * d ** 3, radius=0.5 * d, mat=mat,
+                        gravity=(0.0, 0.0, 0.0))
+    solver, s_out, oout, extras = _run_pair("wcsph", model, cfg, 1.0e-5, oracle)
+    acc = solver._particle_accel.numpy()
+    np.testing.assert_array_equal(acc.view(np.uint32), extras["accel"].view(np.uint32))
+    assert np.abs(acc).max() > 0.0
+    # the other association differs for some pairs
+    f32 = np.float32
+    g = f32(0.0123)
+    m = f32(1000.0 * d ** 3)
+    lib = oracle._lib("f32")
+    partner = np.concatenate([np.arange(npairs) + npairs, np.arange(npairs)])
+    other = np.zeros_like(acc)
+    for i in range(n):
+        dv = pts[i] - pts[partner[i]]
+        dist = np.sqrt((dv[0] * dv[0] + dv[1] * dv[1]) + dv[2] * dv[2])
+        w = f32(lib.ora_cubic_w(lib._real(float(max(dist, f32(0.5 * d)))), lib._real(float(f32(h)))))
+        other[i] = f32(0.0) - ((g / m) * m) * (dv * w)
+    assert np.any(other != acc)
+
+
 def test_dfsph_far_clamped_neighbor_counts(oracle):
     """Every particle active, positions whose query cells lie at or below
     -2^20 on an axis: those cells all hash to bucket 0, so a query visits the
