@@ -358,59 +358,96 @@ void ora_particle_volume(int n, const R* mass, const R* rest_density, R* volume)
     for (int t = 0; t < n; ++t) volume[t] = mass[t] / rmax(rest_density[t], RC(1.0e-6));
 }
 
-/* compute_density_pressure  solver_wcsph.py:119-158 */
+/* compute_density_pressure  solver_wcsph.py:119-158, for particle i */
+static inline void wcsph_density_one(const ora_grid* g, int i, const int* flags, const R* x, const R* rest_density,
+                                     const R* stiffness, const R* exponent, const R* volume, R h,
+                                     int clamp_negative_pressure, R* rho, R* pressure)
+{
+    if (!is_active(flags[i])) { rho[i] = RC(0.0); pressure[i] = RC(0.0); return; }
+    v3 xi = vload(x, i);
+    R rho_temp = RC(0.0);
+    FOR_CANDIDATES(g, xi, h, j, {
+        if (!is_active(flags[j])) continue;
+        v3 dist = vsub(xi, vload(x, j));
+        R mass_nei = rest_density[j] * volume[j];
+        rho_temp += mass_nei * cubic_w(vlen(dist), h);
+    });
+    R rho_i = rho_temp;
+    rho[i] = rho_i;
+    R p = stiffness[i] * (rpow(rho_i / rmax(rest_density[i], RC(1.0e-6)), exponent[i]) - RC(1.0));
+    if (clamp_negative_pressure) p = rmax(p, RC(0.0));
+    pressure[i] = p;
+}
+
 void ora_wcsph_density_pressure(const ora_grid* g, const int* flags, const R* x, const R* rest_density,
                                 const R* stiffness, const R* exponent, const R* volume, R h,
                                 int clamp_negative_pressure, R* rho, R* pressure)
 {
     PARFOR
-    for (int t = 0; t < g->num_points; ++t) {
-        int i = g->point_ids[t];
-        if (!is_active(flags[i])) { rho[i] = RC(0.0); pressure[i] = RC(0.0); continue; }
-        v3 xi = vload(x, i);
-        R rho_temp = RC(0.0);
-        FOR_CANDIDATES(g, xi, h, j, {
-            if (!is_active(flags[j])) continue;
-            v3 dist = vsub(xi, vload(x, j));
-            R mass_nei = rest_density[j] * volume[j];
-            rho_temp += mass_nei * cubic_w(vlen(dist), h);
-        });
-        R rho_i = rho_temp;
-        rho[i] = rho_i;
-        R p = stiffness[i] * (rpow(rho_i / rmax(rest_density[i], RC(1.0e-6)), exponent[i]) - RC(1.0));
-        if (clamp_negative_pressure) p = rmax(p, RC(0.0));
-        pressure[i] = p;
-    }
+    for (int t = 0; t < g->num_points; ++t)
+        wcsph_density_one(g, g->point_ids[t], flags, x, rest_density, stiffness, exponent, volume, h,
+                          clamp_negative_pressure, rho, pressure);
 }
 
-/* acceleration_wcsph  solver_wcsph.py:161-228 */
+/* The same for the listed particles only (rho/pressure written at those
+ * indices): per-particle parity checks of large inputs. */
+void ora_wcsph_density_pressure_ids(const ora_grid* g, int n_query, const int* ids, const int* flags, const R* x,
+                                    const R* rest_density, const R* stiffness, const R* exponent, const R* volume,
+                                    R h, int clamp_negative_pressure, R* rho, R* pressure)
+{
+    PARFOR
+    for (int t = 0; t < n_query; ++t)
+        wcsph_density_one(g, ids[t], flags, x, rest_density, stiffness, exponent, volume, h, clamp_negative_pressure,
+                          rho, pressure);
+}
+
+/* acceleration_wcsph  solver_wcsph.py:161-228, for particle i */
+static inline void wcsph_accel_one(const ora_grid* g, int i, const int* flags, const int* mask, const R* x,
+                                   const R* v, const R* rho, const R* pressure, R particle_radius, const R* mass,
+                                   const R* surface_tension, const R* viscosity, const R* gravity, const int* world,
+                                   R h, R* accel)
+{
+    if (!is_dynamic(flags[i], mask[i])) { vstore(accel, i, VZERO); return; }
+    v3 xi = vload(x, i);
+    R rho_i = rho[i];
+    R p_i = pressure[i];
+    v3 acc = VZERO;
+    FOR_CANDIDATES(g, xi, h, j, {
+        if (j == i || !is_active(flags[j])) continue;
+        v3 direction = vsub(xi, vload(x, j));
+        R distance = vlen(direction);
+        R mass_nei = mass[j];
+        R rho_nei = rho[j];
+        R p_nei = pressure[j];
+        acc = acc_non_pressure(acc, rho_nei, vload(v, i), vload(v, j), mass[i], mass_nei, viscosity[i],
+                               surface_tension[i], direction, rmax(distance, particle_radius), distance, h);
+        acc = acc_pressure(acc, direction, p_i, p_nei, rho_i, rho_nei, mass_nei, h);
+    });
+    int w = world[i] > 0 ? world[i] : 0;
+    vstore(accel, i, vadd(acc, vload(gravity, w)));
+}
+
 void ora_wcsph_acceleration(const ora_grid* g, const int* flags, const int* mask, const R* x, const R* v,
                             const R* rho, const R* pressure, R particle_radius, const R* mass,
                             const R* surface_tension, const R* viscosity, const R* gravity,
                             const int* world, R h, R* accel)
 {
     PARFOR
-    for (int t = 0; t < g->num_points; ++t) {
-        int i = g->point_ids[t];
-        if (!is_dynamic(flags[i], mask[i])) { vstore(accel, i, VZERO); continue; }
-        v3 xi = vload(x, i);
-        R rho_i = rho[i];
-        R p_i = pressure[i];
-        v3 acc = VZERO;
-        FOR_CANDIDATES(g, xi, h, j, {
-            if (j == i || !is_active(flags[j])) continue;
-            v3 direction = vsub(xi, vload(x, j));
-            R distance = vlen(direction);
-            R mass_nei = mass[j];
-            R rho_nei = rho[j];
-            R p_nei = pressure[j];
-            acc = acc_non_pressure(acc, rho_nei, vload(v, i), vload(v, j), mass[i], mass_nei, viscosity[i],
-                                   surface_tension[i], direction, rmax(distance, particle_radius), distance, h);
-            acc = acc_pressure(acc, direction, p_i, p_nei, rho_i, rho_nei, mass_nei, h);
-        });
-        int w = world[i] > 0 ? world[i] : 0;
-        vstore(accel, i, vadd(acc, vload(gravity, w)));
-    }
+    for (int t = 0; t < g->num_points; ++t)
+        wcsph_accel_one(g, g->point_ids[t], flags, mask, x, v, rho, pressure, particle_radius, mass, surface_tension,
+                        viscosity, gravity, world, h, accel);
+}
+
+/* The same for the listed particles only. */
+void ora_wcsph_acceleration_ids(const ora_grid* g, int n_query, const int* ids, const int* flags, const int* mask,
+                                const R* x, const R* v, const R* rho, const R* pressure, R particle_radius,
+                                const R* mass, const R* surface_tension, const R* viscosity, const R* gravity,
+                                const int* world, R h, R* accel)
+{
+    PARFOR
+    for (int t = 0; t < n_query; ++t)
+        wcsph_accel_one(g, ids[t], flags, mask, x, v, rho, pressure, particle_radius, mass, surface_tension,
+                        viscosity, gravity, world, h, accel);
 }
 
 /* integrate_velocity  solver_wcsph.py:231-244 */

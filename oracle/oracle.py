@@ -339,6 +339,40 @@ def wcsph_step(model: OModel, state_in: OState, cfg: WcsphConfig, dt: float, pre
     return out, {"accel": accel, "grid": grid, "volume": vol}
 
 
+def wcsph_passes_ids(model: OModel, x: np.ndarray, v: np.ndarray, cfg: WcsphConfig, ids, rho_full=None,
+                     pressure_full=None, precision="f32", grid=None):
+    """compute_density_pressure and acceleration_wcsph (solver_wcsph.py:119-228)
+    of the listed particles only, on a grid built at ``x`` (or ``grid``).
+    The density pass reads positions only; the acceleration reads the density
+    and pressure of the neighbours, taken from ``rho_full``/``pressure_full``
+    (whole arrays, e.g. another implementation's pass output) or from this
+    call's own density of the listed particles when they are None.  Returns
+    (rho[ids], pressure[ids], accel[ids], grid)."""
+    c = _Ctx(model, precision)
+    m, lib, n = c.m, c.lib, c.m.n
+    f = c.dt_np
+    x = np.ascontiguousarray(x, f)
+    v = np.ascontiguousarray(v, f)
+    ids = np.ascontiguousarray(ids, np.int32)
+    h = smoothing_length(cfg)
+    pr = _particle_radius(cfg, m)
+    vol = _volume(c)
+    grid = grid if grid is not None else c.new_grid().build(x, h)
+    g = grid.struct()
+    rho = np.zeros(n, f)
+    p = np.zeros(n, f)
+    lib.ora_wcsph_density_pressure_ids(ctypes.byref(g), ctypes.c_int(len(ids)), _p(ids), _p(m.flags), _p(x),
+                                       _p(m.rest_density), _p(m.stiffness), _p(m.exponent), _p(vol), c.r(h),
+                                       ctypes.c_int(int(cfg.clamp_negative_pressure)), _p(rho), _p(p))
+    rho_in = rho if rho_full is None else np.ascontiguousarray(rho_full, f)
+    p_in = p if pressure_full is None else np.ascontiguousarray(pressure_full, f)
+    accel = np.zeros((n, 3), f)
+    lib.ora_wcsph_acceleration_ids(ctypes.byref(g), ctypes.c_int(len(ids)), _p(ids), _p(m.flags), _p(m.mask), _p(x),
+                                   _p(v), _p(rho_in), _p(p_in), c.r(pr), _p(m.mass), _p(m.surface_tension),
+                                   _p(m.viscosity), _p(m.gravity), _p(m.world), c.r(h), _p(accel))
+    return rho[ids], p[ids], accel[ids], grid
+
+
 def wcsph_adjoint_step(model: OModel, state_in: OState, cfg: WcsphConfig, dt: float, xbar_out, vbar_out,
                        precision="f32", round_mode=0):
     """Reverse-mode derivative of one ``wcsph_step`` (surface tension must be
