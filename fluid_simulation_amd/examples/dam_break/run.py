@@ -12,11 +12,15 @@ records are the same (:211-227):
   simulation report, completed frames, simulated and elapsed seconds,
   diagnostics, environment, surface, final metrics.
 
-The viewer, the surface extraction and the USD sink are display consumers of
-the positions and are not part of this package: ``--viewer gl``,
-``--show-particles`` and ``--usd`` are refused; ``--no-viewer`` and
-``--no-surface`` are accepted.  ``--device-resident`` runs WCSPH and PBF frames
-with the device-resident driver (no host synchronisation per substep).
+The viewer and the surface extraction (Newton's ``ParticleSurface``) are
+display consumers of the positions and are not part of this package:
+``--viewer gl`` is refused, and ``--usd`` needs ``--no-surface`` (the
+reference then writes the particles alone, surface.py:82-86).  ``--usd PATH``
+(.usd or .usda) writes the particles, time-sampled every ``--usd-every``
+frames plus the final frame, and the tank and height guides as a USD text
+layer (``usd.py``); its record goes into run.json's ``surface`` entry.
+``--device-resident`` runs WCSPH and PBF frames with the device-resident
+driver (no host synchronisation per substep).
 
     python -m fluid_simulation_amd.examples.dam_break.run --solver wcsph --frames 60
 """
@@ -71,7 +75,8 @@ def parse_args(method=None, argv=None):
     parser.add_argument("--no-viewer", dest="viewer", action="store_const", const="none")
     parser.add_argument("--no-surface", action="store_true")
     parser.add_argument("--show-particles", action="store_true")
-    parser.add_argument("--usd")
+    parser.add_argument("--usd", help="Particle animation as a USD text layer (.usd or .usda); needs --no-surface")
+    parser.add_argument("--usd-every", type=positive_int, default=1)
     parser.add_argument("--output-dir", type=Path, default=None, help="Metrics and run metadata directory")
     parser.add_argument("--wcsph-stiffness", type=positive_float, default=None,
                         help="WCSPH Tait pressure constant B in Pa (default: 250000)")
@@ -87,8 +92,14 @@ def parse_args(method=None, argv=None):
     parser.add_argument("--device-resident", action="store_true",
                         help="WCSPH/PBF: device-side CFL and graph-replayed substeps")
     args = parser.parse_args(argv)
-    if args.viewer != "none" or args.show_particles or args.usd:
-        parser.error("the viewer and USD output are not part of this package; use --no-viewer")
+    if args.viewer != "none":
+        parser.error("the viewer is not part of this package; use --no-viewer")
+    if args.usd:
+        if Path(args.usd).suffix.lower() not in (".usd", ".usda"):
+            parser.error("--usd writes a USD text layer: use a .usd or .usda path (.usdc is binary crate)")
+        if not args.no_surface:
+            parser.error("surface meshing (Newton's ParticleSurface) is not part of this package: add --no-surface "
+                         "to write the particles")
     if args.solver != "wcsph" and (
         args.wcsph_stiffness is not None or args.wcsph_viscosity is not None
         or args.wcsph_smoothing_length_coff is not None or args.wcsph_allow_negative_pressure
@@ -147,20 +158,38 @@ def main(method=None, argv=None):
         print(f"WCSPH B={sim.material['stiffness']:g} Pa, viscosity={sim.material['viscosity']:g} m^2/s, "
               f"h={sim.solver.smoothing_length:.6g} m, "
               f"non-negative pressure={sim.solver.config.clamp_negative_pressure}", flush=True)
+    usd = None
+    if args.usd:
+        from .usd import ParticleUsdWriter
+
+        usd = ParticleUsdWriter(args.usd, scene, args.solver)
+
+    def record():
+        if usd is not None:
+            usd.record(sim.frame, sim.state_0.particle_q.numpy())
+
     rows = [sim.metrics()]
     started = time.perf_counter()
-    while sim.frame < args.frames:
-        advance()
-        sim.check_finite()
-        report_now = sim.frame % args.report_every == 0 or sim.frame == args.frames
-        if sim.frame % args.metrics_every == 0 or report_now:
-            metrics = sim.metrics()
-            rows.append(metrics)
-        if report_now:
-            print(f"frame {sim.frame}/{args.frames}, t={sim.sim_time:.4f} s, "
-                  f"substeps={sim.last_substeps}, vmax={metrics['speed_max']:.4g}, finite=True", flush=True)
-    if rows[-1]["frame"] != sim.frame:
-        rows.append(sim.metrics())
+    try:
+        record()
+        while sim.frame < args.frames:
+            advance()
+            sim.check_finite()
+            report_now = sim.frame % args.report_every == 0 or sim.frame == args.frames
+            if sim.frame % args.metrics_every == 0 or report_now:
+                metrics = sim.metrics()
+                rows.append(metrics)
+            if sim.frame % args.usd_every == 0:
+                record()
+            if report_now:
+                print(f"frame {sim.frame}/{args.frames}, t={sim.sim_time:.4f} s, "
+                      f"substeps={sim.last_substeps}, vmax={metrics['speed_max']:.4g}, finite=True", flush=True)
+        record()          # the final state is always in the layer
+        if rows[-1]["frame"] != sim.frame:
+            rows.append(sim.metrics())
+    finally:
+        if usd is not None:
+            usd.close()
     elapsed = time.perf_counter() - started
     simulation = sim.report()
     if args.device_resident:
@@ -168,9 +197,12 @@ def main(method=None, argv=None):
     report = dict(scene=scene.report(), simulation=simulation, completed_frames=sim.frame,
                   simulation_seconds=sim.sim_time, elapsed_seconds=elapsed,
                   diagnostics=dict(metrics_every=args.metrics_every, finite_check_every=1),
-                  environment=environment(sim.model.device), surface=dict(enabled=False), final_metrics=rows[-1])
+                  environment=environment(sim.model.device),
+                  surface=usd.report() if usd is not None else dict(enabled=False), final_metrics=rows[-1])
     write_records(args.output_dir, rows, report)
     print(f"Finished in {elapsed:.2f} s. Metrics: {args.output_dir}", flush=True)
+    if usd is not None:
+        print(f"USD: {usd.path} ({usd.samples} samples)", flush=True)
     return report
 
 

@@ -38,3 +38,22 @@ def test_run_writes_records(tmp_path, method, extra):
     assert loaded["final_metrics"]["frame"] == 4
     assert report["completed_frames"] == 4
     assert np.isfinite(float(rows[-1]["height_mean"]))
+
+
+def test_run_writes_particle_usd(tmp_path):
+    """--usd with --no-surface: the particles every --usd-every frames and at
+    the final frame, in the layer's time samples (surface.py:156-160)."""
+    from fluid_simulation_amd.examples.dam_break.usd import read_points_samples
+
+    out = tmp_path / "wcsph"
+    usd = tmp_path / "anim" / "dam.usd"
+    report = run.main("wcsph", ["--particles", "2000", "--frames", "5", "--usd", str(usd), "--no-surface",
+                                "--usd-every", "2", "--output-dir", str(out)])
+    samples = read_points_samples(usd)
+    assert sorted(samples) == [0, 2, 4, 5]
+    np.testing.assert_array_equal(samples[0], make_scene(2000).positions)
+    assert all(np.isfinite(x).all() and x.shape == (make_scene(2000).count, 3) for x in samples.values())
+    assert not np.array_equal(samples[5], samples[0])
+    surf = json.loads((out / "run.json").read_text())["surface"]
+    assert surf["usd_samples"] == 4 and surf["usd_path"] == str(usd.resolve())
+    assert report["surface"]["frame_samples"][-1]["frame"] == 5

@@ -17,8 +17,11 @@ def test_cli_defaults_and_validation(tmp_path):
     assert (args.solver, args.frames, args.substeps, args.particles) == ("wcsph", 300, 4, 400000)
     assert (args.report_every, args.metrics_every) == (10, 10)
     assert run.parse_args("pbf", argv=["--no-viewer", "--no-surface"]).solver == "pbf"
+    a = run.parse_args(argv=["--usd", "x.usda", "--no-surface", "--usd-every", "5", "--show-particles"])
+    assert (a.usd, a.usd_every) == ("x.usda", 5)
     for bad in (["--solver", "pbf", "--wcsph-stiffness", "1e5"], ["--solver", "dfsph", "--wcsph-allow-negative-pressure"],
-                ["--viewer", "gl"], ["--usd", "x.usd"], ["--solver", "dfsph", "--device-resident"],
+                ["--viewer", "gl"], ["--usd", "x.usd"], ["--usd", "x.usdc", "--no-surface"],
+                ["--solver", "dfsph", "--device-resident"],
                 ["--frames", "0"], ["--wcsph-viscosity", "-1"], ["--solver", "apic"]):
         with pytest.raises(SystemExit):
             run.parse_args(argv=bad)
