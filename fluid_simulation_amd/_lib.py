@@ -79,7 +79,7 @@ class FsDfsphParams(ctypes.Structure):
         ("smoothing_length", c_float), ("particle_radius", c_float), ("bound", c_float * 3),
         ("boundary_damping", c_float), ("dt", c_float), ("max_kappa", c_float), ("max_kappa_v", c_float),
         ("max_rho_iterations", c_int32), ("max_vel_iterations", c_int32), ("warm_start", c_int32),
-        ("min_neighbors_for_factor", c_int32), ("exact_residual", c_int32),
+        ("min_neighbors_for_factor", c_int32), ("exact_residual", c_int32), ("dt_device", c_void_p),
     ]
 
 
@@ -89,6 +89,8 @@ class FsDfsphIO(ctypes.Structure):
         ("x_out", c_void_p), ("v_out", c_void_p), ("rho", c_void_p), ("factor", c_void_p),
         ("rho_star", c_void_p), ("d_rho_dt", c_void_p), ("kappa", c_void_p), ("kappa_v", c_void_p),
         ("neighbor_count", c_void_p), ("v_star", c_void_p), ("accel", c_void_p), ("iterations", c_void_p),
+        ("rho_in", c_void_p), ("factor_in", c_void_p), ("rho_star_in", c_void_p), ("d_rho_dt_in", c_void_p),
+        ("neighbor_count_in", c_void_p), ("v_star_in", c_void_p),
     ]
 
 
@@ -202,6 +204,8 @@ SIGNATURES = {
                                       POINTER(FsBatchIO), c_void_p]),
     "fs_batch_cfl_dt": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_double, c_double,
                                   c_double, c_double, c_double, c_int32, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fs_dfsph_cfl_dt": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_double, c_double, c_int32, c_void_p,
+                                  c_void_p, c_void_p, c_void_p]),
     "fs_batch_cfl_dt_noalloc": (c_int32, [c_void_p, c_int32, c_int32, c_void_p, c_void_p, c_int32, c_double,
                                           c_double, c_double, c_double, c_double, c_int32, c_void_p, c_void_p,
                                           c_void_p]),

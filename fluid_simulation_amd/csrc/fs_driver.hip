@@ -214,6 +214,30 @@ static int32_t launch_cfl(fs_scene* scenes, int32_t num_scenes, int32_t max_scen
     return FS_OK;
 }
 
+// DamBreakSimulation.step's DFSPH dt policy per scene (simulation.py:166-187):
+// limit = frame_dt/substeps, and 0.4 spacing / vmax when vmax > 0
+__global__ void k_scene_dt_dfsph(fs_scene* scenes, int num_scenes, uint32_t* maxima, double spacing, double frame_dt,
+                                 int substeps, double* remaining, int* status)
+{
+    const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sidx >= num_scenes) return;
+    const float v2 = __uint_as_float(maxima[2 * sidx]);
+    maxima[2 * sidx] = 0u;
+    maxima[2 * sidx + 1] = 0u;
+    double rem = remaining[sidx];
+    double dt = 0.0;
+    if (rem > 1.0e-12) {
+        if (!isfinite(v2)) atomicMax(status, 1);
+        const double speed = sqrt((double)v2);
+        double limit = frame_dt / substeps;
+        if (speed > 0.0) limit = fmin(limit, 0.4 * spacing / speed);
+        dt = fmin(rem, limit);
+        if (dt < 1.0e-10) atomicMax(status, 2);
+        remaining[sidx] = rem - dt;
+    }
+    scenes[sidx].dt = (float)dt;
+}
+
 static int red_blocks(int n)
 {
     int b = blocks_for(n, kRedBlock * 8);
@@ -255,6 +279,24 @@ int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_
     }
     return launch_cfl(scenes, num_scenes, max_scene_particles, v, a, use_accel, stiffness, exponent, rest_density,
                       viscosity, frame_dt, substeps, remaining, scratch, status, (hipStream_t)stream);
+}
+
+int32_t fs_dfsph_cfl_dt(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_particles, const float* v,
+                        double spacing, double frame_dt, int32_t substeps, double* remaining, uint32_t* scratch,
+                        int32_t* status, fs_stream_t stream)
+{
+    int32_t rc = check_cfl_args("fs_dfsph_cfl_dt", scenes, num_scenes, v, nullptr, 0, remaining, status, substeps);
+    if (rc) return rc;
+    if (!scratch) {
+        set_error("fs_dfsph_cfl_dt: invalid arguments (null scratch)");
+        return FS_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int per_scene = max_scene_particles > 0 ? std::min(128, blocks_for(max_scene_particles, kRedBlock * 8)) : 8;
+    FS_LAUNCH(k_scene_maxima, dim3(per_scene, num_scenes), dim3(kRedBlock), 0, s, scenes, v, nullptr, 0, scratch);
+    FS_LAUNCH(k_scene_dt_dfsph, dim3(blocks_for(num_scenes, 64)), dim3(64), 0, s, scenes, num_scenes, scratch, spacing,
+              frame_dt, substeps, remaining, status);
+    return FS_OK;
 }
 
 int32_t fs_batch_cfl_dt_noalloc(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_particles, const float* v,

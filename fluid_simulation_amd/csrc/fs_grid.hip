@@ -137,10 +137,11 @@ template <int MODE>
 __device__ __forceinline__ v3 pre_point(const PreOp& p, int i)
 {
     v3 x = ld3(p.x_in, i);
+    const float dt = p.dt_device ? *p.dt_device : p.dt;
     if (MODE == kPreIntegrate) {
         // integrate_positions  solver_dfsph.py:340-353
         bool dyn = (p.flags[i] & 1) != 0 && p.mask[i] == 0;
-        if (dyn) x = vadd(x, muls(ld3(p.v_in, i), p.dt));
+        if (dyn) x = vadd(x, muls(ld3(p.v_in, i), dt));
         st3(p.x_out, i, x);
     } else if (MODE == kPrePredict) {
         bool dyn = (p.flags[i] & 1) != 0 && p.mask[i] == 0;
@@ -149,9 +150,9 @@ __device__ __forceinline__ v3 pre_point(const PreOp& p, int i)
             // predict_velocity_with_gravity  solver_pbf.py:192-208
             int w = p.world[i];
             w = w > 0 ? w : 0;
-            v = vadd(v, smul(p.dt, ld3(p.gravity, w)));
+            v = vadd(v, smul(dt, ld3(p.gravity, w)));
             // predict_positions  solver_pbf.py:211-224
-            x = vadd(x, smul(p.dt, v));
+            x = vadd(x, smul(dt, v));
             // apply_position_bounds  solver_pbf.py:366-399
             float r = p.particle_radius, e = p.boundary_epsilon;
             float x_min = -p.bound[0] + r + e, x_max = p.bound[0] - r - e;

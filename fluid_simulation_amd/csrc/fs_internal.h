@@ -26,6 +26,7 @@ struct PreOp {
     const int* world = nullptr;
     const float* gravity = nullptr;
     float dt = 0.0f;
+    const float* dt_device = nullptr;   // replaces dt when set
     float bound[3] = {0, 0, 0};
     float particle_radius = 0.0f;
     float boundary_epsilon = 0.0f;

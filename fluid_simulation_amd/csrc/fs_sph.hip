@@ -41,6 +41,15 @@ enum Control : int {
     kControlWords = 64,
 };
 
+// The substep's dt: the device scalar when one is given (a device dt of 0
+// marks an idle substep, which the caller's driver uses once a frame's time
+// is spent), else the launch parameter.  Returns false for an idle substep.
+__device__ __forceinline__ bool substep_dt(const float* dt_device, float host_dt, float& dt)
+{
+    dt = dt_device ? *dt_device : host_dt;
+    return !(dt_device && dt == 0.0f);
+}
+
 struct PassCommon {
     GridView g;
     int n;
@@ -202,6 +211,7 @@ struct ResidualArgs {
     const float* __restrict__ sub;
     float* scratch;
     int force_exact;   // decide every iteration with the exact float32 mean
+    const float* dt_device;   // idle substep: no iteration
 };
 
 // One block.  The rate pass's float64 partial sums give the mean to about
@@ -222,6 +232,8 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_residual(ResidualArgs a)
     __shared__ float s_leaf_sum[2 * kNpBuffer / kNpLeaf + 8];
     __shared__ int s_nleaf;
     if (a.control[a.stop_idx]) return;
+    float dt_unused;
+    if (!substep_dt(a.dt_device, 1.0f, dt_unused)) return;
     double tv = 0.0, tc = 0.0;
     for (int b = threadIdx.x; b < a.nb; b += kPassBlock) {
         tv += a.partials[2 * b];
@@ -817,6 +829,7 @@ struct DfDensityArgs {
     float4* sca;   // rho, factor, kappa, kappa/max(rho,1e-6)
     float4* vel;   // .w <- m/max(rho,1e-6)
     const int* any_inactive;   // set by the grid build when some particle is inactive
+    const float* dt_device;    // idle substep (device dt 0): nothing to do
 };
 
 // compute_density + compute_factor_dfsph  solver_dfsph.py:106-174 (one pass)
@@ -826,6 +839,8 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
     __shared__ int lds[BLOCK * CAP];
     const int s = pass_slot_b<BLOCK>();
     if (s >= a.c.n) return;
+    float dt_unused;
+    if (!substep_dt(a.dt_device, 1.0f, dt_unused)) return;
     const int i = a.c.perm[s];
     const float4 xi4 = a.c.pos[s];
     if (!(a.c.sflags[s] & kActive)) {
@@ -899,6 +914,7 @@ struct DfNonPressureArgs {
     const int* __restrict__ world;
     float particle_radius;
     float dt;
+    const float* dt_device;
     float* accel;
     float* v_star;
     float4* vstar_s;
@@ -911,6 +927,8 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureA
     __shared__ int lds[kPassBlock * CAP];
     const int s = pass_slot();
     if (s >= a.c.n) return;
+    float dt;
+    if (!substep_dt(a.dt_device, a.dt, dt)) return;
     const int i = a.c.perm[s];
     if (!(a.c.sflags[s] & kDynamic)) {
         st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
@@ -948,7 +966,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureA
     w = w > 0 ? w : 0;
     const v3 acc_g = vadd(acc, ld3(a.gravity, w));
     st3(a.accel, i, acc_g);
-    const v3 vs = vadd(vi, smul(a.dt, acc_g));
+    const v3 vs = vadd(vi, smul(dt, acc_g));
     st3(a.v_star, i, vs);
     a.vstar_s[s] = make_float4(vs.x, vs.y, vs.z, 0.0f);
 }
@@ -958,6 +976,7 @@ struct DfRateArgs {
     const float4* __restrict__ vstar_s;
     const float* __restrict__ rest_density;
     float dt;
+    const float* dt_device;
     float max_kappa;
     int density_loop;   // 1: density loop (rho_star, kappa), 0: divergence loop (kappa_v)
     float* d_rho_dt;
@@ -976,6 +995,8 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
     __shared__ int lds[kPassBlock * CAP];
     const int stop_idx = a.density_loop ? kStopRho : kStopVel;
     if (a.control[stop_idx]) return;
+    float dt;
+    if (!substep_dt(a.dt_device, a.dt, dt)) return;
     const int s = pass_slot();
     double res_v = 0.0, res_c = 0.0;
     if (s < a.c.n) {
@@ -1005,21 +1026,21 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
             const bool dyn = (fl & kDynamic) != 0;
             float kap = 0.0f;
             if (a.density_loop) {
-                const float rs = si.x + a.dt * div_val;
+                const float rs = si.x + dt * div_val;
                 a.rho_star[i] = rs;
                 const float diff = rs - a.rest_density[i];
                 const float err = fmax_(diff, 0.0f);
                 if (dyn) {
                     res_v = (double)np_max0(diff);
                     res_c = 1.0;
-                    kap = fmin_((err / (a.dt * a.dt)) * si.y, a.max_kappa);
+                    kap = fmin_((err / (dt * dt)) * si.y, a.max_kappa);
                 }
             } else {
                 const float dc = fmax_(div_val, 0.0f);
                 if (dyn) {
                     res_v = (double)np_max0(div_val);
                     res_c = 1.0;
-                    kap = fmin_((1.0f / a.dt) * dc * si.x * si.y, a.max_kappa);
+                    kap = fmin_((1.0f / dt) * dc * si.x * si.y, a.max_kappa);
                 }
             }
             si.z = kap;
@@ -1034,6 +1055,7 @@ struct DfCorrArgs {
     PassCommon c;
     const float4* __restrict__ sca;
     float dt;
+    const float* dt_device;
     int stop_idx;   // -1: unconditional (warm start)
     int corr_idx;
     float* v_star;
@@ -1049,6 +1071,8 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_correction(DfCorrArgs a)
 {
     __shared__ int lds[kPassBlock * CAP];
     if (a.stop_idx >= 0 && a.control[a.stop_idx]) return;
+    float dt;
+    if (!substep_dt(a.dt_device, a.dt, dt)) return;
     const int s = pass_slot();
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.corr_idx >= 0) a.control[a.corr_idx] += 1;
     if (s >= a.c.n) return;
@@ -1069,7 +1093,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_correction(DfCorrArgs a)
             dv = vadd(dv, smul(coeff, cubic_grad(vsub(xi, xyz(xj)), h)));
         }));
     const float4 vs4 = a.vstar_s[s];
-    const v3 vs = vsub(xyz(vs4), smul(a.dt, dv));
+    const v3 vs = vsub(xyz(vs4), smul(dt, dv));
     st3(a.v_star, i, vs);
     a.vstar_s[s] = make_float4(vs.x, vs.y, vs.z, 0.0f);
 }
@@ -1085,14 +1109,52 @@ struct DfFinalArgs {
     float damping;
     const int* control;
     int* iterations;
+    // idle substep (device dt 0): carry the input state over
+    const float* dt_device;
+};
+
+// state_in -> state_out of an idle substep (pointers may be null)
+struct DfCarry {
+    const float* x_in;
+    const float* v_in;
+    const float* rho_in;
+    const float* factor_in;
+    const float* rho_star_in;
+    const float* d_rho_dt_in;
+    const float* kappa_in;
+    const float* kappa_v_in;
+    const int* neighbor_count_in;
+    const float* v_star_in;
+    float* rho;
+    float* factor;
+    float* rho_star;
+    float* d_rho_dt;
+    float* kappa;
+    float* kappa_v;
+    int* neighbor_count;
+    float* v_star;
 };
 
 // copy v_star -> particle_qd, apply_bounds (DFSPH variant)  solver_dfsph.py:406-442,885-898
-__global__ __launch_bounds__(kElemBlock) void k_dfsph_finalize(DfFinalArgs a)
+__global__ __launch_bounds__(kElemBlock) void k_dfsph_finalize(DfFinalArgs a, DfCarry c)
 {
     const int t = blockIdx.x * kElemBlock + threadIdx.x;
     if (a.iterations && t < 4) a.iterations[t] = a.control[kItRho + t];
     if (t >= a.n) return;
+    float dt_unused;
+    if (!substep_dt(a.dt_device, 1.0f, dt_unused)) {
+        st3(a.x_out, t, ld3(c.x_in, t));
+        st3(a.v_out, t, ld3(c.v_in, t));
+        if (c.rho_in) c.rho[t] = c.rho_in[t];
+        if (c.factor_in) c.factor[t] = c.factor_in[t];
+        if (c.rho_star_in) c.rho_star[t] = c.rho_star_in[t];
+        if (c.d_rho_dt_in) c.d_rho_dt[t] = c.d_rho_dt_in[t];
+        if (c.kappa_in) c.kappa[t] = c.kappa_in[t];
+        if (c.kappa_v_in) c.kappa_v[t] = c.kappa_v_in[t];
+        if (c.neighbor_count_in) c.neighbor_count[t] = c.neighbor_count_in[t];
+        if (c.v_star_in) st3(c.v_star, t, ld3(c.v_star_in, t));
+        return;
+    }
     v3 v = ld3(a.v_star, t);
     if (!((a.flags[t] & 1) != 0 && a.mask[t] == 0)) {
         st3(a.v_out, t, v);
@@ -2108,6 +2170,7 @@ static int32_t df_density_factor(const DfPlan& p, bool after_integrate)
     dfa.neighbor_count = io->neighbor_count;
     dfa.sca = p.sca;
     dfa.vel = p.vel;
+    dfa.dt_device = p.prm->dt_device;
     FS_LAUNCH((k_dfsph_density_factor<kScanBlock, kScanCap>), dim3(blocks_for(p.n, kScanBlock)), dim3(kScanBlock), 0,
               p.s, dfa);
     return FS_OK;
@@ -2125,6 +2188,7 @@ static int32_t df_nonpressure_predict(const DfPlan& p)
     npa.world = p.m->world;
     npa.particle_radius = p.prm->particle_radius;
     npa.dt = p.dt;
+    npa.dt_device = p.prm->dt_device;
     npa.accel = p.io->accel;
     npa.v_star = p.io->v_star;
     npa.vstar_s = p.vstar_s;
@@ -2142,6 +2206,7 @@ static int32_t df_correction(const DfPlan& p, int loop, bool warm)
     ca.c = make_common(p.g, p.w, p.pos, p.h, kListUse);
     ca.sca = p.sca;
     ca.dt = p.dt;
+    ca.dt_device = p.prm->dt_device;
     ca.v_star = p.io->v_star;
     ca.vstar_s = p.vstar_s;
     ca.control = p.w->control;
@@ -2168,6 +2233,7 @@ static int32_t df_rate_kappa(const DfPlan& p, int loop)
     ra.vstar_s = p.vstar_s;
     ra.rest_density = p.m->rest_density;
     ra.dt = p.dt;
+    ra.dt_device = p.prm->dt_device;
     ra.d_rho_dt = p.io->d_rho_dt;
     ra.rho_star = p.io->rho_star;
     ra.sca = p.sca;
@@ -2198,6 +2264,7 @@ static int32_t df_residual(const DfPlan& p, int loop, int iteration)
     ra.sub = loop == 0 ? p.m->rest_density : nullptr;
     ra.scratch = p.w->x_pred;   // unused by DFSPH otherwise; holds >= n floats
     ra.force_exact = p.prm->exact_residual;
+    ra.dt_device = p.prm->dt_device;
     FS_LAUNCH(k_dfsph_residual, dim3(1), dim3(kPassBlock), 0, p.s, ra);
     return FS_OK;
 }
@@ -2215,6 +2282,7 @@ static int32_t df_integrate(const DfPlan& p)
     pre2.flags = m->flags;
     pre2.mask = m->mask;
     pre2.dt = p.dt;
+    pre2.dt_device = p.prm->dt_device;
     GatherOp ga2 = gather_for(m, p.w, io->x_out, io->v_star, p.vstar_s);
     ga2.any_inactive = p.w->control + kAnyInactive;   // same particles: sets the same flag again
     return build_grid(p.g, p.n, p.h, pre2, ga2, p.s);
@@ -2234,7 +2302,28 @@ static int32_t df_finalize(const DfPlan& p)
     fa.damping = p.prm->boundary_damping;
     fa.control = p.w->control;
     fa.iterations = p.io->iterations;
-    FS_LAUNCH(k_dfsph_finalize, dim3(blocks_for(p.n, kElemBlock)), dim3(kElemBlock), 0, p.s, fa);
+    fa.dt_device = p.prm->dt_device;
+    const fs_dfsph_io* io = p.io;
+    DfCarry c;
+    c.x_in = io->x_in;
+    c.v_in = io->v_in;
+    c.rho_in = io->rho_in;
+    c.factor_in = io->factor_in;
+    c.rho_star_in = io->rho_star_in;
+    c.d_rho_dt_in = io->d_rho_dt_in;
+    c.kappa_in = io->kappa_in;
+    c.kappa_v_in = io->kappa_v_in;
+    c.neighbor_count_in = io->neighbor_count_in;
+    c.v_star_in = io->v_star_in;
+    c.rho = io->rho;
+    c.factor = io->factor;
+    c.rho_star = io->rho_star;
+    c.d_rho_dt = io->d_rho_dt;
+    c.kappa = io->kappa;
+    c.kappa_v = io->kappa_v;
+    c.neighbor_count = io->neighbor_count;
+    c.v_star = io->v_star;
+    FS_LAUNCH(k_dfsph_finalize, dim3(blocks_for(p.n, kElemBlock)), dim3(kElemBlock), 0, p.s, fa, c);
     return FS_OK;
 }
 

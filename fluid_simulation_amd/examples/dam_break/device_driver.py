@@ -11,9 +11,12 @@ the host before every WCSPH/DFSPH substep.  Here:
   dt = 0, which leaves the state, density, pressure and acceleration as they
   are.  The host reads one flag per chunk to learn whether the frame is
   complete.
+* DFSPH does the same with its own policy (``fs_dfsph_cfl_dt``: frame/4,
+  lowered to 0.4 spacing / vmax).  Its passes read dt from device memory too,
+  and a substep with dt = 0 is idle: it copies the whole input state (x, v,
+  and the ``sph`` arrays) to the output.  Its substeps are long and few per
+  frame, so its chunks are short (4 substeps).
 * PBF has a fixed substep (frame/substeps), so a whole frame is one graph.
-* DFSPH keeps the host driver: its dt enters many of its passes as a launch
-  parameter.
 """
 from __future__ import annotations
 
@@ -27,12 +30,15 @@ from .simulation import DamBreakSimulation
 
 
 class DeviceResidentDamBreak:
-    def __init__(self, scene, method: str = "wcsph", device="cuda:0", substeps: int = 4, chunk: int = 16,
+    def __init__(self, scene, method: str = "wcsph", device="cuda:0", substeps: int = 4, chunk: int | None = None,
                  **sim_kwargs):
         """``sim_kwargs`` go to ``DamBreakSimulation`` (the WCSPH material and
-        pressure-clamp options)."""
-        if method not in ("wcsph", "pbf"):
-            raise ValueError("device-resident frames cover wcsph and pbf; dfsph uses DamBreakSimulation")
+        pressure-clamp options).  ``chunk``: substeps per graph replay
+        (default 16 for WCSPH, 4 for DFSPH)."""
+        if method not in ("wcsph", "dfsph", "pbf"):
+            raise ValueError(f"unknown method {method!r}")
+        if chunk is None:
+            chunk = 4 if method == "dfsph" else 16
         if chunk < 2 or chunk % 2:
             raise ValueError("chunk must be even (the substeps swap two state buffers)")
         self.method = method
@@ -45,7 +51,7 @@ class DeviceResidentDamBreak:
         self._graph = None
         self.sim = DamBreakSimulation(scene, method, device=device, substeps=substeps, **sim_kwargs)
         self.frame_dt = self.sim.frame_dt
-        if method == "wcsph":
+        if method in ("wcsph", "dfsph"):
             self._cfl_setup()
 
     # ---- state views ----------------------------------------------------------
@@ -80,6 +86,14 @@ class DeviceResidentDamBreak:
     def _cfl(self):
         sim = self.sim
         mat = sim.material
+        if self.method == "dfsph":
+            _lib.check(_lib.load().fs_dfsph_cfl_dt(
+                ctypes.c_void_p(self._table.data_ptr()), 1, self.scene.count,
+                ctypes.c_void_p(sim.state_0.particle_qd.ptr), float(self.scene.spacing), self.frame_dt,
+                self.substeps, ctypes.c_void_p(self.remaining.data_ptr()),
+                ctypes.c_void_p(self._scratch.data_ptr()), ctypes.c_void_p(self.status.data_ptr()),
+                ctypes.c_void_p(_lib.stream_handle(self.device))), "fs_dfsph_cfl_dt")
+            return
         _lib.check(_lib.load().fs_batch_cfl_dt(
             ctypes.c_void_p(self._table.data_ptr()), 1, self.scene.count,
             ctypes.c_void_p(sim.state_0.particle_qd.ptr), ctypes.c_void_p(sim.solver._particle_accel.ptr),
@@ -89,13 +103,13 @@ class DeviceResidentDamBreak:
             ctypes.c_void_p(self.status.data_ptr()), ctypes.c_void_p(_lib.stream_handle(self.device))),
             "fs_batch_cfl_dt")
 
-    def _wcsph_substep(self):
+    def _cfl_substep(self):
         self._cfl()
         self.sim._substep(self.dt)
 
-    def _wcsph_chunk(self):
+    def _cfl_chunk(self):
         for _ in range(self.chunk):
-            self._wcsph_substep()
+            self._cfl_substep()
 
     def _pbf_frame(self):
         dt = self.frame_dt / self.substeps
@@ -132,12 +146,12 @@ class DeviceResidentDamBreak:
             return self.substeps
         self.remaining.fill_(self.frame_dt)
         taken = 0
-        if sim.total_substeps == 0:
-            self._wcsph_substep()              # the first substep has no acceleration term yet
+        if sim.total_substeps == 0 and self.method == "wcsph":
+            self._cfl_substep()                # the first substep has no acceleration term yet
             taken += 1
         if self._graph is None:
             total = sim.total_substeps
-            self._graph = self._capture(self._wcsph_chunk)
+            self._graph = self._capture(self._cfl_chunk)
             sim.total_substeps = total             # capture only recorded the chunk
         while bool((self.remaining > 1.0e-12).item()):
             self._graph.replay()
@@ -145,7 +159,8 @@ class DeviceResidentDamBreak:
             taken += self.chunk
             code = int(self.status.item())
             if code == 1:
-                raise RuntimeError("Non-finite velocity or WCSPH acceleration")
+                raise RuntimeError("Non-finite velocity or WCSPH acceleration" if self.method == "wcsph"
+                                   else f"Non-finite velocity at frame {self.frame}")
             if code == 2 or taken > 4096:
                 raise RuntimeError("CFL step cannot safely complete this frame")
         self._end_frame(taken)

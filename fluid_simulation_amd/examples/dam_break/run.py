@@ -19,8 +19,8 @@ reference then writes the particles alone, surface.py:82-86).  ``--usd PATH``
 (.usd or .usda) writes the particles, time-sampled every ``--usd-every``
 frames plus the final frame, and the tank and height guides as a USD text
 layer (``usd.py``); its record goes into run.json's ``surface`` entry.
-``--device-resident`` runs WCSPH and PBF frames with the device-resident
-driver (no host synchronisation per substep).
+``--device-resident`` runs frames with the device-resident driver (no host
+synchronisation per substep).
 
     python -m fluid_simulation_amd.examples.dam_break.run --solver wcsph --frames 60
 """
@@ -90,7 +90,7 @@ def parse_args(method=None, argv=None):
     parser.add_argument("--metrics-every", type=positive_int, default=10,
                         help="Full statistics interval; finite-state checks still run every frame")
     parser.add_argument("--device-resident", action="store_true",
-                        help="WCSPH/PBF: device-side CFL and graph-replayed substeps")
+                        help="device-side CFL and graph-replayed substeps")
     args = parser.parse_args(argv)
     if args.viewer != "none":
         parser.error("the viewer is not part of this package; use --no-viewer")
@@ -105,8 +105,6 @@ def parse_args(method=None, argv=None):
         or args.wcsph_smoothing_length_coff is not None or args.wcsph_allow_negative_pressure
     ):
         parser.error("--wcsph-* options apply to WCSPH")
-    if args.device_resident and args.solver == "dfsph":
-        parser.error("--device-resident applies to WCSPH and PBF")
     args.output_dir = args.output_dir or Path.cwd() / "output" / args.solver
     return args
 

@@ -86,7 +86,16 @@ class SolverDFSPH(ParticleSolver):
         p.particle_radius = self._particle_radius()
         p.bound[:] = self._bounds
         p.boundary_damping = float(cfg.boundary_damping)
-        p.dt = float(dt)
+        if isinstance(dt, torch.Tensor):
+            # a device scalar (float32, one element), as for SolverWCSPH: a
+            # device-side dt policy and graph capture drive whole frames; a
+            # device dt of 0 is an idle substep that carries the state over
+            if dt.dtype != torch.float32 or dt.numel() != 1 or dt.device.type != "cuda":
+                raise ValueError("a tensor dt must be one float32 on the solver's device")
+            p.dt = 0.0
+            p.dt_device = dt.data_ptr()
+        else:
+            p.dt = float(dt)
         p.max_kappa = float(cfg.max_kappa)
         p.max_kappa_v = float(cfg.max_kappa_v)
         p.max_rho_iterations = int(cfg.max_rho_iterations)
@@ -112,6 +121,12 @@ class SolverDFSPH(ParticleSolver):
         io.v_star = ptr(so.v_star)
         io.accel = ptr(self._particle_accel)
         io.iterations = self._iterations.data_ptr()
+        io.rho_in = ptr(si.rho)
+        io.factor_in = ptr(si.factor)
+        io.rho_star_in = ptr(si.rho_star)
+        io.d_rho_dt_in = ptr(si.d_rho_dt)
+        io.neighbor_count_in = ptr(si.neighbor_count)
+        io.v_star_in = ptr(si.v_star)
         m = self._model_struct()
         _lib.check(_lib.load().fs_dfsph_step(ctypes.byref(self.model.particle_grid._c), ctypes.byref(self._work),
                                              ctypes.byref(m), ctypes.byref(p), ctypes.byref(io),

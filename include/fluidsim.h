@@ -300,6 +300,12 @@ typedef struct fs_dfsph_params {
                                   test (slow; for checks).  0: only when the
                                   float64 mean lies within float32 rounding
                                   of eta                                      */
+    /* Optional device scalar that replaces `dt` (NULL: use `dt`), as for
+     * WCSPH: a device-side CFL (fs_dfsph_cfl_dt) and graph capture then
+     * drive whole frames.  A device dt of 0 makes the substep idle: it
+     * copies x_in, v_in and the *_in state arrays that are set to the
+     * outputs and changes nothing else. */
+    const float* dt_device;
 } fs_dfsph_params;
 
 typedef struct fs_dfsph_io {
@@ -321,6 +327,14 @@ typedef struct fs_dfsph_io {
     int32_t* iterations;       /* device int32[4]: density iterations,
                                   density corrections, divergence iterations,
                                   divergence corrections (may be NULL) */
+    /* Optional (may be NULL): the state_in.sph arrays an idle substep (device
+     * dt 0) carries over to the outputs, with kappa_in / kappa_v_in. */
+    const float* rho_in;
+    const float* factor_in;
+    const float* rho_star_in;
+    const float* d_rho_dt_in;
+    const int32_t* neighbor_count_in;
+    const float* v_star_in;    /* [n*3] */
 } fs_dfsph_io;
 
 /* One DFSPH substep (SolverDFSPH.step, solver_dfsph.py:598-898).  The
@@ -521,6 +535,15 @@ int32_t fs_batch_wcsph_step(fs_grid* grid, fs_work* work, const fs_model* model,
 int32_t fs_batch_cfl_dt(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_particles, const float* v,
                         const float* a, int32_t use_accel, double stiffness, double exponent, double rest_density,
                         double viscosity, double frame_dt, int32_t substeps, double* remaining, uint32_t* scratch,
+                        int32_t* status, fs_stream_t stream);
+
+/* The DFSPH dt policy of the dam-break driver (simulation.py:166-187) per
+ * scene, on the device: limit = frame_dt/substeps, lowered to
+ * 0.4 spacing/vmax when vmax > 0; scenes[s].dt = min(remaining[s], limit)
+ * while remaining[s] > 1e-12 (else 0), remaining[s] -= dt.  status and
+ * scratch as for fs_batch_cfl_dt. */
+int32_t fs_dfsph_cfl_dt(fs_scene* scenes, int32_t num_scenes, int32_t max_scene_particles, const float* v,
+                        double spacing, double frame_dt, int32_t substeps, double* remaining, uint32_t* scratch,
                         int32_t* status, fs_stream_t stream);
 
 /* fs_batch_cfl_dt without the scratch argument; it allocates nothing.  No

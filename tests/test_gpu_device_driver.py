@@ -1,6 +1,7 @@
 """Device-resident dam-break frames match the host driver bit for bit: WCSPH
-(one-scene batch, device CFL in float64 with the host's float64 h,
-graph-replayed chunks) and PBF (one graph per frame)."""
+and DFSPH (one-scene device CFL in float64 with the host's float64 h or
+spacing, device dt in every pass, graph-replayed chunks) and PBF (one graph
+per frame)."""
 import numpy as np
 import pytest
 
@@ -44,6 +45,26 @@ def test_pbf_frames_match_the_host_driver_bitwise():
     np.testing.assert_array_equal(dev.velocities.cpu().numpy(), host.state_0.particle_qd.numpy())
 
 
-def test_dfsph_is_refused():
+def test_dfsph_frames_match_the_host_driver_bitwise():
+    """DFSPH with the device CFL (0.4 spacing / vmax) and the device dt in
+    every pass: the same states as the host driver, bit for bit, with the
+    idle substeps after each frame's end carrying the whole state over."""
+    scene = make_scene(10000)
+    host = DamBreakSimulation(scene, "dfsph", device="cuda:0")
+    dev = DeviceResidentDamBreak(scene, "dfsph", device="cuda:0")
+    for frame in range(3):
+        host.step()
+        taken = dev.step()
+        print(f"frame {frame}: host {host.last_substeps} substeps, device {taken} (incl. idle)", flush=True)
+        assert taken >= host.last_substeps
+        hs, ds = host.state_0, dev.sim.state_0
+        np.testing.assert_array_equal(ds.particle_q.numpy(), hs.particle_q.numpy())
+        np.testing.assert_array_equal(ds.particle_qd.numpy(), hs.particle_qd.numpy())
+        for name in ("rho", "factor", "rho_star", "d_rho_dt", "kappa", "kappa_v", "neighbor_count", "v_star"):
+            np.testing.assert_array_equal(getattr(ds.sph, name).numpy(), getattr(hs.sph, name).numpy(),
+                                          err_msg=name)
+
+
+def test_unknown_method_is_refused():
     with pytest.raises(ValueError):
-        DeviceResidentDamBreak(make_scene(512), "dfsph", device="cuda:0")
+        DeviceResidentDamBreak(make_scene(512), "apic", device="cuda:0")

@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("method,extra", [("wcsph", []), ("dfsph", []), ("pbf", []), ("wcsph", ["--device-resident"]),
-                                          ("pbf", ["--device-resident"])])
+                                          ("pbf", ["--device-resident"]),
+                                          ("dfsph", ["--device-resident"])])
 def test_run_writes_records(tmp_path, method, extra):
     out = tmp_path / method
     report = run.main(method, ["--particles", "2000", "--frames", "4", "--metrics-every", "2", "--report-every", "4",
@@ -25,7 +26,7 @@ def test_run_writes_records(tmp_path, method, extra):
     assert all(r["finite"] == "True" for r in rows)
     substeps = [int(r["substeps"]) for r in rows[1:]]
     assert all(s >= 4 for s in substeps)
-    if method != "wcsph" or not extra:
+    if method == "pbf" or not extra:
         assert int(rows[-1]["total_substeps"]) >= 2 * substeps[-1]
     assert float(rows[-1]["density_mean"]) > 0.0
     loaded = json.loads((out / "run.json").read_text())

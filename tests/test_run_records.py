@@ -21,7 +21,6 @@ def test_cli_defaults_and_validation(tmp_path):
     assert (a.usd, a.usd_every) == ("x.usda", 5)
     for bad in (["--solver", "pbf", "--wcsph-stiffness", "1e5"], ["--solver", "dfsph", "--wcsph-allow-negative-pressure"],
                 ["--viewer", "gl"], ["--usd", "x.usd"], ["--usd", "x.usdc", "--no-surface"],
-                ["--solver", "dfsph", "--device-resident"],
                 ["--frames", "0"], ["--wcsph-viscosity", "-1"], ["--solver", "apic"]):
         with pytest.raises(SystemExit):
             run.parse_args(argv=bad)
