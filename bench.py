@@ -136,6 +136,8 @@ def parse():
     ap.add_argument("--adjoint", action="store_true", help="run only the differentiable WCSPH form (50,540 "
                                                             "particles, 200 substeps) with the FP64 FD check")
     ap.add_argument("--no-fd", action="store_true", help="skip the FD check of --adjoint")
+    ap.add_argument("--adjoint-no-clamp", action="store_true", help="--adjoint with p = B((rho/rho0)^g - 1) "
+                                                                    "unclamped (no max(p, 0) branch)")
     ap.add_argument("--frames", type=int, default=0, help="run only the whole-frame comparison (host CFL driver "
                                                           "vs device-resident) over this many frames")
     ap.add_argument("--separation", action="store_true", help="run only the APIC particle separation "
@@ -212,15 +214,16 @@ def bench_solver(method, scene, steps, warmup, tune=True, settle_frames=0):
         sim._substep(driver_dt(sim))
     dt = driver_dt(sim)
     tuned = None
-    if method == "wcsph" and tune:
+    if tune:
+        from fluid_simulation_amd.autotune import SOLVER_PASSES
         from fluid_simulation_amd.autotune import tune as tune_passes
 
         # variants give identical results: time them on scratch output state
         s_scratch = sim.model.state()
-        tuned = tune_passes(lambda: sim.solver.step(sim.state_0, s_scratch, None, None, dt),
-                            ("wcsph_density", "wcsph_force"))
-        print("# wcsph autotune: " + ", ".join(f"{k} -> block {v['block']} depth {v['list_depth']}"
-                                               for k, v in tuned.items()), file=sys.stderr, flush=True)
+        tuned = tune_passes(lambda: sim.solver.step(sim.state_0, s_scratch, None, None, dt), SOLVER_PASSES[method])
+        print(f"# {method} autotune: " + ", ".join(
+            f"{k} -> block {v['block']} depth {v['list_depth']} unroll {v['unroll']} ppt {v['particles_per_thread']}"
+            for k, v in tuned.items()), file=sys.stderr, flush=True)
     state = None
     if method == "wcsph":
         # the timed substeps start here: the CPU baseline restarts from this state
@@ -253,8 +256,9 @@ def bench_solver(method, scene, steps, warmup, tune=True, settle_frames=0):
         # host enqueue gaps as well; their per-kernel split is the rocprof one
         out["kernel_times"] = f"profiles/{method}_dam_break_401856_kernel_stats.md"
     if tuned:
-        out["autotune"] = {k: {"block": v["block"], "list_depth": v["list_depth"],
-                               "us_per_launch_by_variant": v["us_per_launch"]} for k, v in tuned.items()}
+        out["autotune"] = {k: {"block": v["block"], "list_depth": v["list_depth"], "unroll": v["unroll"],
+                               "particles_per_thread": v["particles_per_thread"],
+                               "us_per_step_by_variant": v["us_per_launch"]} for k, v in tuned.items()}
     if method == "dfsph":
         out["iterations_last_substep"] = iters
         b = dfsph_bytes(iters["density_iterations"], iters["density_corrections"], iters["divergence_iterations"],
@@ -478,14 +482,15 @@ def batched(num_scenes, steps, warmup, chunk=10):
                        "particle_substeps_per_s_sequential_eager": total / seq_eager,
                        "speedup_vs_sequential_eager": seq_eager / graphed,
                        "speedup_vs_sequential_graph": seq_graph / graphed,
-                       "autotune": {k: {"block": v["block"], "list_depth": v["list_depth"],
+                       "autotune": {k: {"block": v["block"], "list_depth": v["list_depth"], "unroll": v["unroll"],
+                                        "particles_per_thread": v["particles_per_thread"],
                                         "us_per_launch_by_variant": v["us_per_launch"]} for k, v in tuned.items()}}}
     del graph, sims, batch
     torch.cuda.empty_cache()
     return out
 
 
-def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_scale=0.05):
+def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_scale=0.05, clamp=True):
     """Differentiable WCSPH: the 50,540-particle column (x in [-1.2,-0.6],
     y in [-0.28,0.28], z in [0,0.6], jitter 0.01 s), seeded random initial
     velocities (``v_scale`` m/s rms per axis), 200 substeps at dt = 0.25 h/c0,
@@ -496,7 +501,10 @@ def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_
     central differences (steps 1e-4 and 1e-5 relative) of d/dstiffness,
     d/dviscosity and a random direction of d/dv0; (b) the GPU gradients against
     the FP32 oracle's adjoint of the same FP32 rollout; (c) the GPU gradients
-    against the FP64 differences directly."""
+    against the FP64 differences directly.  ``clamp`` is the EOS clamp
+    p = max(p, 0) (the dam break's setting); the check also counts the
+    particle-substeps whose clamp branch (rho > rho0) differs between the
+    FP32 and the FP64 rollouts."""
     import numpy as np
     import torch
 
@@ -511,7 +519,7 @@ def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_
     xtarget = float(np.quantile(x0n[:, 0], 0.95)) + float(rng.normal(scale=0.02))
     dt = dt_for(scene.spacing)
     sim = DifferentiableWCSPH(x0n, scene.mass, scene.radius, scene.spacing, dt=dt, steps=substeps,
-                              checkpoint_every=checkpoint_every)
+                              checkpoint_every=checkpoint_every, clamp_negative_pressure=clamp)
     x0 = torch.from_numpy(x0n).cuda()
     v0d = torch.from_numpy(v0n).cuda()
     ct, xt = torch.from_numpy(ctarget).cuda(), torch.tensor(xtarget, dtype=torch.float64).cuda()
@@ -543,6 +551,7 @@ def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_
     torch.cuda.synchronize()
     t_both = (time.perf_counter() - t0) / reps
     out = {"particles": scene.count, "substeps": substeps, "checkpoint_every": checkpoint_every, "dt": dt,
+           "clamp_negative_pressure": clamp,
            "stiffness": B0, "viscosity": NU0, "v0_rms": v_scale, "loss": loss, "dL_dstiffness": gB,
            "dL_dviscosity": gnu, "ms_forward": t_fwd * 1e3, "ms_forward_backward": t_both * 1e3,
            "backward_over_forward": (t_both - t_fwd) / t_fwd,
@@ -557,7 +566,7 @@ def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_
         oracle.set_threads(threads)
         cfg = oracle.WcsphConfig(particle_radius=scene.radius, particle_length=scene.spacing,
                                  smoothing_length_coff=2.0, bound_width=1.2, bound_height=0.28, bound_length=0.6,
-                                 boundary_damping=0.0)
+                                 boundary_damping=0.0, clamp_negative_pressure=clamp)
         dv = np.random.default_rng(12).normal(size=v0n.shape)
         rel = lambda a, b: abs(a - b) / max(abs(b), 1e-300)
         t0 = time.perf_counter()
@@ -579,11 +588,18 @@ def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_
             g[order[kq + 1], 0] += 2.0 * (p95 - xtarget) * fr
             return float(((c - ctarget) ** 2).sum() + (p95 - xtarget) ** 2), g
 
+        rhos = {}
+        kept = {}
+
         def oracle_adjoint(prec):
             npd = np.float64 if prec == "f64" else np.float32
-            states = oracle.wcsph_rollout(model(npd), x0n.astype(npd), v0n.astype(npd), cfg, dt, substeps, prec)
+            rhos[prec] = []
+            states = oracle.wcsph_rollout(model(npd), x0n.astype(npd), v0n.astype(npd), cfg, dt, substeps, prec,
+                                          densities=rhos[prec])
             _, g = loss_and_grad(states[-1][0].astype(np.float64))
             _, vb, gs, gvv = oracle.wcsph_rollout_adjoint(model(npd), states, cfg, dt, g, np.zeros_like(g), prec)
+            if prec == "f32":
+                kept["states"], kept["seed"] = states, g
             return float(gs.sum()), float(gvv.sum()), np.asarray(vb, np.float64)
 
         a64 = oracle_adjoint("f64")
@@ -606,7 +622,22 @@ def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_
         a32 = oracle_adjoint("f32")
         print(f"# adjoint check: fp32 oracle adjoint done ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
               flush=True)
+        # the FP64 adjoint linearised along the FP32 trajectory (its stored
+        # states, each substep recomputed in FP64): the derivative of the FP32
+        # trajectory's branch pattern without FP32 arithmetic in the adjoint
+        st64 = [(x.astype(np.float64), v.astype(np.float64)) for x, v in kept.pop("states")]
+        _, vbm, gsm, gvm = oracle.wcsph_rollout_adjoint(model(np.float64), st64, cfg, dt, kept["seed"],
+                                                        np.zeros_like(kept["seed"]), "f64")
+        del st64
+        mixed = (float(gsm.sum()), float(gvm.sum()), float((np.asarray(vbm, np.float64) * dv).sum()))
+        print(f"# adjoint check: fp64 adjoint along the fp32 trajectory done ({time.perf_counter() - t0:.0f} s)",
+              file=sys.stderr, flush=True)
         names = ("stiffness", "viscosity", "v0_direction")
+        # particle-substeps on different sides of rho0 in the two precisions:
+        # there p = max(B((rho/rho0)^g - 1), 0) takes different branches
+        flips = sum(int(np.count_nonzero((r32 > np.float32(1000.0)) != (r64 > 1000.0)))
+                    for r32, r64 in zip(rhos["f32"], rhos["f64"]))
+        compressed = sum(int(np.count_nonzero(r64 > 1000.0)) for r64 in rhos["f64"])
         ad64 = (a64[0], a64[1], float((a64[2] * dv).sum()))
         gpu = (gB, gnu, float((gv * dv).sum()))
         check = {"step_relative": list(fds), "host_threads": threads,
@@ -618,6 +649,10 @@ def adjoint(target=50000, substeps=200, checkpoint_every=10, fd=True, reps=3, v_
                                                                              np.abs(a32[2]).max())},
                  "gpu_vs_fp64_fd": {n: [gpu[k]] + [fds[e][k] for e in fds] +
                                     [min(rel(gpu[k], fds[e][k]) for e in fds)] for k, n in enumerate(names)},
+                 "fp64_adjoint_along_fp32_trajectory_vs_gpu": {n: [mixed[k], gpu[k], rel(gpu[k], mixed[k])]
+                                                               for k, n in enumerate(names)},
+                 "clamp_branch_disagreements_fp32_vs_fp64": flips,
+                 "particle_substeps_above_rest_density_fp64": compressed,
                  "format": "[gradient, fd step 1e-4, fd step 1e-5, smaller relative error]",
                  "seconds": time.perf_counter() - t0}
         out["fd_check"] = check
@@ -751,7 +786,7 @@ def main():
         print(json.dumps(frames(count=args.frames)), flush=True)
         return
     if args.adjoint:
-        print(json.dumps(adjoint(fd=not args.no_fd)), flush=True)
+        print(json.dumps(adjoint(fd=not args.no_fd, clamp=not args.adjoint_no_clamp)), flush=True)
         return
     if args.batched:
         print(json.dumps(batched(args.batched, args.steps, args.warmup)), flush=True)

@@ -145,6 +145,8 @@ SIGNATURES = {
     "fs_profile_report": (c_int32, [ctypes.c_char_p, c_size_t]),
     "fs_tune_variant_count": (c_int32, [c_int32]),
     "fs_tune_describe": (c_int32, [c_int32, c_int32, POINTER(c_int32), POINTER(c_int32)]),
+    "fs_tune_describe_ex": (c_int32, [c_int32, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
+                                      POINTER(c_int32)]),
     "fs_tune_select": (c_int32, [c_int32, c_int32]),
     "fs_tune_selected": (c_int32, [c_int32]),
     "fs_grid_workspace_bytes": (c_size_t, [c_int32, c_int32, c_int32, c_int32]),

@@ -1,12 +1,14 @@
 """Autotuner for the launch variants of the tunable neighbour passes.
 
-The library exposes, per tunable pass, a few launch variants: workgroup size
-x depth of the per-lane LDS survivor list. The list depth sets how many
-workgroups fit in a CU's LDS, and so the occupancy. All variants give
-bit-identical results. ``tune`` times every variant of each requested pass on
-the caller's own workload, using the library's per-kernel HIP-event timing,
-and selects the fastest. ``autotune`` does the same behind a JSON cache keyed
-by device and workload.
+The library exposes, per tunable pass, a few launch variants: workgroup size,
+depth of the per-lane LDS survivor list (it sets how many workgroups fit in a
+CU's LDS, and so the occupancy), the number of neighbours a list replay
+fetches together and the particles per lane.  All variants give bit-identical
+results.  ``tune`` times every variant of each requested pass on the caller's
+own workload, using the library's per-kernel HIP-event timing, and selects
+the fastest.  ``autotune`` does the same behind a JSON cache keyed by the GPU
+architecture and the workload (``workload_key``: solver and particle-count
+bucket).
 """
 from __future__ import annotations
 
@@ -19,21 +21,43 @@ import torch
 
 from . import _lib
 
-PASSES = {"wcsph_density": 0, "wcsph_force": 1, "batch_density": 2, "batch_force": 3}
-KERNEL = {"wcsph_density": "k_wcsph_density", "wcsph_force": "k_wcsph_force",
-          "batch_density": "k_wcsph_density", "batch_force": "k_wcsph_force"}
+PASSES = {"wcsph_density": 0, "wcsph_force": 1, "batch_density": 2, "batch_force": 3, "dfsph_scan": 4,
+          "dfsph_replay": 5, "pbf_scan": 6, "pbf_replay": 7}
+# the kernels whose time a pass's variant changes
+KERNEL = {"wcsph_density": ("k_wcsph_density",), "wcsph_force": ("k_wcsph_force",),
+          "batch_density": ("k_wcsph_density",), "batch_force": ("k_wcsph_force",),
+          "dfsph_scan": ("k_dfsph_density_factor",),
+          "dfsph_replay": ("k_dfsph_nonpressure", "k_dfsph_rate", "k_dfsph_correction"),
+          "pbf_scan": ("k_pbf_density", "k_pbf_lambda"),
+          "pbf_replay": ("k_pbf_nonpressure", "k_pbf_delta", "k_pbf_xsph")}
+# the passes each solver's substep can tune
+SOLVER_PASSES = {"wcsph": ("wcsph_density", "wcsph_force"), "dfsph": ("dfsph_scan", "dfsph_replay"),
+                 "pbf": ("pbf_scan", "pbf_replay")}
 
 
-def variants(pass_name: str) -> list[tuple[int, int]]:
-    """(workgroup size, LDS list depth) of every variant of a pass."""
+def describe(pass_name: str) -> list[dict]:
+    """Every variant of a pass: workgroup size, LDS list depth, replay unroll
+    and particles per lane."""
     lib = _lib.load()
     pid = PASSES[pass_name]
     out = []
     for v in range(lib.fs_tune_variant_count(pid)):
-        b, d = ctypes.c_int32(), ctypes.c_int32()
-        _lib.check(lib.fs_tune_describe(pid, v, ctypes.byref(b), ctypes.byref(d)), "fs_tune_describe")
-        out.append((b.value, d.value))
+        b, d, u, p = (ctypes.c_int32() for _ in range(4))
+        _lib.check(lib.fs_tune_describe_ex(pid, v, ctypes.byref(b), ctypes.byref(d), ctypes.byref(u),
+                                           ctypes.byref(p)), "fs_tune_describe_ex")
+        out.append({"block": b.value, "list_depth": d.value, "unroll": u.value, "particles_per_thread": p.value})
     return out
+
+
+def variants(pass_name: str) -> list[tuple[int, int]]:
+    """(workgroup size, LDS list depth) of every variant of a pass."""
+    return [(v["block"], v["list_depth"]) for v in describe(pass_name)]
+
+
+def workload_key(solver: str, n: int) -> str:
+    """Cache key of a workload: the solver and its particle count rounded up
+    to a power of two."""
+    return f"{solver}-n{1 << max(0, int(n) - 1).bit_length()}"
 
 
 def select(pass_name: str, variant: int) -> None:
@@ -44,10 +68,10 @@ def selected(pass_name: str) -> int:
     return int(_lib.load().fs_tune_selected(PASSES[pass_name]))
 
 
-def _kernel_ms(name: str) -> float:
+def _kernel_ms(names) -> float:
     total = 0.0
     for k, (_, ms) in _lib.profile_report().items():
-        if re.sub(r"<.*>", "", k.strip("()")) == name:
+        if re.sub(r"<.*>", "", k.strip("()")) in names:
             total += ms
     return total
 
@@ -55,8 +79,10 @@ def _kernel_ms(name: str) -> float:
 def tune(step, passes, *, warmup: int = 2, reps: int = 5) -> dict:
     """Time ``step()`` (enqueues one substep on the current stream) under
     every variant of each pass in ``passes``, one pass at a time, and select
-    the variant with the least time in that pass's kernel.  Returns
-    {pass: {"variant", "block", "list_depth", "us_per_launch": [...]}}."""
+    the variant with the least time in that pass's kernels.  Returns
+    {pass: {"variant", "block", "list_depth", "unroll", "particles_per_thread",
+    "us_per_launch": [...]}} (us_per_launch: the pass's kernel time per
+    step, per variant)."""
     lib = _lib.load()
     result = {}
     for name in passes:
@@ -74,10 +100,14 @@ def tune(step, passes, *, warmup: int = 2, reps: int = 5) -> dict:
             times.append(1e3 * _kernel_ms(KERNEL[name]) / reps)
         best = min(range(len(times)), key=times.__getitem__)
         select(name, best)
-        b, d = variants(name)[best]
-        result[name] = {"variant": best, "block": b, "list_depth": d, "us_per_launch": times}
+        result[name] = dict(variant=best, **describe(name)[best], us_per_launch=times)
     lib.fs_profile_reset()
     return result
+
+
+def device_arch() -> str:
+    """The GPU architecture name (e.g. gfx950...), the cache's device key."""
+    return torch.cuda.get_device_properties(0).gcnArchName
 
 
 def cache_path() -> str:
@@ -94,8 +124,9 @@ def _load_cache() -> dict:
 
 
 def autotune(step, passes, workload: str, *, use_cache: bool = True, **kw) -> dict:
-    """``tune`` behind a cache keyed by device name and ``workload``."""
-    key = f"{torch.cuda.get_device_name()}|{workload}"
+    """``tune`` behind a cache keyed by the GPU architecture and ``workload``
+    (e.g. ``workload_key(solver, n)``)."""
+    key = f"{device_arch()}|{workload}"
     cache = _load_cache() if use_cache else {}
     hit = cache.get(key)
     if hit and all(p in hit for p in passes):

@@ -24,7 +24,7 @@ ARCH = os.environ.get("FLUIDSIM_ARCH", "gfx950")
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
-    "-std=c++17",
+    "-std=c++20",
     "-fPIC",
     "-shared",
     "-ffp-contract=off",

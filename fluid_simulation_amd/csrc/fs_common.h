@@ -434,6 +434,7 @@ __device__ __forceinline__ void run_listed(Body& body, int k)
     else body(k);
 }
 constexpr int kPhasedUnroll = 4;   // neighbours whose data is fetched together
+constexpr int kReplayUnroll = 4;   // ... in the list-replaying loop
 
 // Visit the candidates of hash_grid_query(p, r) in Warp order (returning how
 // many there were), but call
@@ -567,7 +568,7 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
 // no list or it overflowed (the lean scan); a scanning pass always scans with
 // the primary loop and records when its list is in build mode.  The role is
 // a template parameter so that each kernel carries only the loop it runs.
-template <int BLOCK, int CAP, bool REPLAY, int BATCH = kBatch, class Count, class Body>
+template <int BLOCK, int CAP, bool REPLAY, int BATCH = kBatch, int UNR = kReplayUnroll, class Count, class Body>
 __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4* __restrict__ pos, int s, float px,
                                                   float py, float pz, float r, float cut2, int* __restrict__ lds,
                                                   const NbrList& L, Count&& count, Body&& body)
@@ -582,37 +583,46 @@ __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4
                 const ptrdiff_t n = L.n;
                 int e = 0;
                 if constexpr (IsPhased<typename std::decay<Body>::type>::value) {
-                    // software-pipelined by one group: the next four indices
-                    // are issued before this group's neighbour data, so a
-                    // group costs one round trip instead of two (long lists
-                    // are latency chains)
-                    if (m >= 4) {
-                        int k0 = idx[0], k1 = idx[n], k2 = idx[2 * n], k3 = idx[3 * n];
-                        idx += 4 * n;
-                        for (e = 4;; e += 4) {
-                            const bool more = e + 4 <= m;
-                            int n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-                            if (more) {
-                                n0 = idx[0];
-                                n1 = idx[n];
-                                n2 = idx[2 * n];
-                                n3 = idx[3 * n];
-                                idx += 4 * n;
-                            }
-                            const auto d0 = load_listed(body, k0);
-                            const auto d1 = load_listed(body, k1);
-                            const auto d2 = load_listed(body, k2);
-                            const auto d3 = load_listed(body, k3);
-                            body.body(k0, d0);
-                            body.body(k1, d1);
-                            body.body(k2, d2);
-                            body.body(k3, d3);
+                    // software-pipelined by one group: the next group's
+                    // indices are issued before this group's neighbour data,
+                    // so a group costs one round trip instead of two (long
+                    // lists are latency chains)
+                    constexpr int U = UNR;   // neighbours whose data is fetched together
+                    if (m >= U) {
+                        int k[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) k[u] = idx[u * n];
+                        idx += U * n;
+                        for (e = U;; e += U) {
+                            const bool more = e + U <= m;
+                            int nk[U];
+#pragma unroll
+                            for (int u = 0; u < U; ++u) nk[u] = more ? idx[u * n] : 0;
+                            if (more) idx += U * n;
+                            decltype(load_listed(body, 0)) d[U];
+#pragma unroll
+                            for (int u = 0; u < U; ++u) d[u] = load_listed(body, k[u]);
+#pragma unroll
+                            for (int u = 0; u < U; ++u) body.body(k[u], d[u]);
                             if (!more) break;
-                            k0 = n0;
-                            k1 = n1;
-                            k2 = n2;
-                            k3 = n3;
+#pragma unroll
+                            for (int u = 0; u < U; ++u) k[u] = nk[u];
                         }
+                    }
+                    // the last partial group in one round trip: slots past the
+                    // list's end fetch the particle's own data and are skipped
+                    const int r = m - e;
+                    if (r > 0) {
+                        int k[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) k[u] = u < r ? idx[u * n] : s;
+                        decltype(load_listed(body, 0)) d[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u) d[u] = load_listed(body, k[u]);
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                            if (u < r) body.body(k[u], d[u]);
+                        e = m;
                     }
                 } else {
                     for (; e + 4 <= m; e += 4) {

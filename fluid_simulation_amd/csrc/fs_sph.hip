@@ -23,8 +23,8 @@
 namespace fs {
 
 constexpr int kPassBlock = 256;
-constexpr int kScanBlock = 64;  // workgroup of the untuned scanning passes (DFSPH, PBF)
-constexpr int kScanCap = 48;    // LDS survivor list of a scanning pass (DFSPH, PBF)
+constexpr int kMinPassBlock = 64;   // smallest workgroup of a tunable pass
+constexpr int kScanBlock = 64;  // workgroup of the separation pass
 constexpr int kSepCap = 24;     // separation: few survivors, a shorter list keeps occupancy
 constexpr int kReplayCap = 16;  // ... of a list-replaying pass (only its overflow fallback scans)
 constexpr float kCutSlack = 1.0001f;   // candidates with d2 > h^2*slack have q > 1
@@ -112,22 +112,25 @@ __device__ __forceinline__ void scene_view(PassCommon& c, const fs_scene& sc)
 
 // neighbour loop of a scanning pass (SCAN_*) or of a list-replaying pass
 // (REPLAY_*); the _B forms are for kernels templated on their block size BLOCK
-#define NBR_LOOP(blk, rep, c, s, px, py, pz, count, body)                                                \
-    for_each_neighbor<blk, CAP, rep>((c).g, (c).pos, (s), (px), (py), (pz), (c).h, (c).cut2, lds, (c).L, \
-                                     count, body)
-#define SCAN_NEIGHBORS(...) NBR_LOOP(kPassBlock, false, __VA_ARGS__)
-#define REPLAY_NEIGHBORS(...) NBR_LOOP(kPassBlock, true, __VA_ARGS__)
-#define SCAN_NEIGHBORS_B(...) NBR_LOOP(BLOCK, false, __VA_ARGS__)
-#define REPLAY_NEIGHBORS_B(...) NBR_LOOP(BLOCK, true, __VA_ARGS__)
+#define NBR_LOOP(blk, rep, unr, c, s, px, py, pz, count, body)                                                 \
+    for_each_neighbor<blk, CAP, rep, kBatch, unr>((c).g, (c).pos, (s), (px), (py), (pz), (c).h, (c).cut2, lds,  \
+                                                  (c).L, count, body)
+#define SCAN_NEIGHBORS(...) NBR_LOOP(kPassBlock, false, kReplayUnroll, __VA_ARGS__)
+#define REPLAY_NEIGHBORS(...) NBR_LOOP(kPassBlock, true, kReplayUnroll, __VA_ARGS__)
+#define SCAN_NEIGHBORS_B(...) NBR_LOOP(BLOCK, false, kReplayUnroll, __VA_ARGS__)
+#define REPLAY_NEIGHBORS_B(...) NBR_LOOP(BLOCK, true, kReplayUnroll, __VA_ARGS__)
+// a replaying pass templated on its block size and replay unroll (UNR)
+#define REPLAY_NEIGHBORS_BU(...) NBR_LOOP(BLOCK, true, UNR, __VA_ARGS__)
 
 // ---------------------------------------------------------------------------
 // DFSPH residual: each block of the rate pass stores its partial sums; a
 // one-block kernel then reduces them in a fixed order and decides the early exit
 // ---------------------------------------------------------------------------
+template <int BLOCK>
 __device__ void residual_partials(double v, double c, double* partials)
 {
-    __shared__ double s_v[kPassBlock / 64];
-    __shared__ double s_c[kPassBlock / 64];
+    __shared__ double s_v[BLOCK / 64];
+    __shared__ double s_c[BLOCK / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -141,7 +144,7 @@ __device__ void residual_partials(double v, double c, double* partials)
     __syncthreads();
     if (threadIdx.x == 0) {
         double bv = 0.0, bc = 0.0;
-        for (int w = 0; w < kPassBlock / 64; ++w) {
+        for (int w = 0; w < BLOCK / 64; ++w) {
             bv += s_v[w];
             bc += s_c[w];
         }
@@ -521,7 +524,7 @@ struct WcsphForceArgs {
 // p/max(rho^2,1e-6), m/max(rho,1e-6) precomputed -- the same float operations
 // on the same operands.  A zero cohesion (gamma) or viscosity (mu) coefficient
 // contributes an exact zero, so its term is skipped.
-template <int BLOCK, int CAP>
+template <int BLOCK, int CAP, int UNR>
 __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const PassCommon& c, float pr, float dt,
                                                  float dmp, float sx, float sy, float sz, int s, int* lds)
 {
@@ -542,7 +545,7 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
     const float visc_den0 = 0.01f * (h * h);
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
-    REPLAY_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+    REPLAY_NEIGHBORS_BU(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
         [&](int k) { return NbrXVS{c.sflags[k], c.pos[k], a.vel[k], a.sca[k].z}; },
         [&](int k, const NbrXVS& nb) {
             if (k == s || !(nb.flags & kActive)) return;
@@ -577,35 +580,40 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
     st3(a.x_out, i, x);
 }
 
-template <int BLOCK, int CAP, bool BATCH>
+// PPT particles per lane: the workgroup covers PPT consecutive tiles of BLOCK
+// slots and each lane takes one slot of each, one after another
+template <int BLOCK, int CAP, bool BATCH, int UNR, int PPT>
 __global__ __launch_bounds__(BLOCK) void k_wcsph_force(WcsphForceArgs a)
 {
     __shared__ int lds[BLOCK * CAP];
-    const int s = pass_slot_b<BLOCK>();
-    if (s >= a.c.n) return;
-    if constexpr (BATCH) {
-        FOR_MY_SCENE(a, s, sc, {
-            if (sc.dt != 0.0f) {
-                PassCommon c = a.c;
-                scene_view(c, sc);
-                wcsph_force_lane<BLOCK, CAP>(a, c, sc.particle_radius, sc.dt, sc.boundary_damping, sc.bound[0],
-                                             sc.bound[1], sc.bound[2], s, lds);
-            } else {   // idle scene: carry the state over, keep its acceleration
+    const int tile = xcd_tile();
+    for (int q = 0; q < PPT; ++q) {
+        const int s = (tile * PPT + q) * BLOCK + threadIdx.x;
+        if (s >= a.c.n) return;
+        if constexpr (BATCH) {
+            FOR_MY_SCENE(a, s, sc, {
+                if (sc.dt != 0.0f) {
+                    PassCommon c = a.c;
+                    scene_view(c, sc);
+                    wcsph_force_lane<BLOCK, CAP, UNR>(a, c, sc.particle_radius, sc.dt, sc.boundary_damping,
+                                                      sc.bound[0], sc.bound[1], sc.bound[2], s, lds);
+                } else {   // idle scene: carry the state over, keep its acceleration
+                    const int i = a.c.perm[s];
+                    st3(a.v_out, i, ld3(a.v_in, i));
+                    st3(a.x_out, i, ld3(a.x_in, i));
+                }
+            });
+        } else {
+            const float dt = a.dt_device ? *a.dt_device : a.dt;
+            if (dt == 0.0f) {   // idle substep (device dt 0): carry the state over
                 const int i = a.c.perm[s];
                 st3(a.v_out, i, ld3(a.v_in, i));
                 st3(a.x_out, i, ld3(a.x_in, i));
+                continue;
             }
-        });
-    } else {
-        const float dt = a.dt_device ? *a.dt_device : a.dt;
-        if (dt == 0.0f) {   // idle substep (device dt 0): carry the state over
-            const int i = a.c.perm[s];
-            st3(a.v_out, i, ld3(a.v_in, i));
-            st3(a.x_out, i, ld3(a.x_in, i));
-            return;
+            wcsph_force_lane<BLOCK, CAP, UNR>(a, a.c, a.particle_radius, dt, a.damping, a.bound[0], a.bound[1],
+                                              a.bound[2], s, lds);
         }
-        wcsph_force_lane<BLOCK, CAP>(a, a.c, a.particle_radius, dt, a.damping, a.bound[0], a.bound[1], a.bound[2], s,
-                                     lds);
     }
 }
 
@@ -921,11 +929,11 @@ struct DfNonPressureArgs {
 };
 
 // acceleration_non_pressure + predict_velocity  solver_dfsph.py:177-242
-template <int CAP>
-__global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureArgs a)
+template <int BLOCK, int CAP, int UNR>
+__global__ __launch_bounds__(BLOCK) void k_dfsph_nonpressure(DfNonPressureArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
-    const int s = pass_slot();
+    __shared__ int lds[BLOCK * CAP];
+    const int s = pass_slot_b<BLOCK>();
     if (s >= a.c.n) return;
     float dt;
     if (!substep_dt(a.dt_device, a.dt, dt)) return;
@@ -945,7 +953,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_nonpressure(DfNonPressureA
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
     if (gamma != 0.0f || mu != 0.0f) {
-        REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+        REPLAY_NEIGHBORS_BU(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
             [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vel[k]}; },
             [&](int k, const NbrXV& nb) {
                 if (k == s || !(nb.flags & kActive)) return;
@@ -989,15 +997,15 @@ struct DfRateArgs {
 // compute_density_change_rate (+ predict_density_star, compute_kappa_density
 // or compute_kappa_divergence, and the host residual)  solver_dfsph.py:245-306,
 // 356-372, 579-596
-template <int CAP>
-__global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
+template <int BLOCK, int CAP, int UNR>
+__global__ __launch_bounds__(BLOCK) void k_dfsph_rate(DfRateArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
+    __shared__ int lds[BLOCK * CAP];
     const int stop_idx = a.density_loop ? kStopRho : kStopVel;
     if (a.control[stop_idx]) return;
     float dt;
     if (!substep_dt(a.dt_device, a.dt, dt)) return;
-    const int s = pass_slot();
+    const int s = pass_slot_b<BLOCK>();
     double res_v = 0.0, res_c = 0.0;
     if (s < a.c.n) {
         const int i = a.c.perm[s];
@@ -1014,7 +1022,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
             const v3 vi = xyz(a.vstar_s[s]);
             const float h = a.c.h;
             float div_val = 0.0f;
-            REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+            REPLAY_NEIGHBORS_BU(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
                 [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vstar_s[k]}; },
                 [&](int k, const NbrXV& nb) {
                     if (!(nb.flags & kActive)) return;
@@ -1048,7 +1056,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_rate(DfRateArgs a)
             a.sca[s] = si;
         }
     }
-    residual_partials(res_v, res_c, a.partials);
+    residual_partials<BLOCK>(res_v, res_c, a.partials);
 }
 
 struct DfCorrArgs {
@@ -1066,14 +1074,14 @@ struct DfCorrArgs {
 
 // apply_constant_density_correction / apply_divergence_free_correction
 // solver_dfsph.py:309-337, 375-403 (and the kappa write of :290-306, 356-372)
-template <int CAP>
-__global__ __launch_bounds__(kPassBlock) void k_dfsph_correction(DfCorrArgs a)
+template <int BLOCK, int CAP, int UNR>
+__global__ __launch_bounds__(BLOCK) void k_dfsph_correction(DfCorrArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
+    __shared__ int lds[BLOCK * CAP];
     if (a.stop_idx >= 0 && a.control[a.stop_idx]) return;
     float dt;
     if (!substep_dt(a.dt_device, a.dt, dt)) return;
-    const int s = pass_slot();
+    const int s = pass_slot_b<BLOCK>();
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.corr_idx >= 0) a.control[a.corr_idx] += 1;
     if (s >= a.c.n) return;
     const int i = a.c.perm[s];
@@ -1084,7 +1092,7 @@ __global__ __launch_bounds__(kPassBlock) void k_dfsph_correction(DfCorrArgs a)
     const float h = a.c.h;
     const float kri = si.w;
     v3 dv = V(0.0f, 0.0f, 0.0f);
-    REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+    REPLAY_NEIGHBORS_BU(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
         [&](int k) { return NbrXS{a.c.sflags[k], a.c.pos[k], a.sca[k].w}; },
         [&](int k, const NbrXS& nb) {
             if (!(nb.flags & kActive)) return;
@@ -1292,11 +1300,11 @@ struct PbfNonPressureArgs {
 
 // acceleration_non_pressure (PBF) + apply_non_pressure_prediction +
 // apply_position_bounds  solver_pbf.py:144-189, 227-242, 784-796
-template <int CAP>
-__global__ __launch_bounds__(kPassBlock) void k_pbf_nonpressure(PbfNonPressureArgs a)
+template <int BLOCK, int CAP, int UNR>
+__global__ __launch_bounds__(BLOCK) void k_pbf_nonpressure(PbfNonPressureArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
-    const int s = pass_slot();
+    __shared__ int lds[BLOCK * CAP];
+    const int s = pass_slot_b<BLOCK>();
     if (s < a.c.n) {
         const int i = a.c.perm[s];
         const float4 xi4 = a.c.pos[s];
@@ -1313,7 +1321,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_nonpressure(PbfNonPressureAr
             const float mu10 = 10.0f * mu;
             v3 acc = V(0.0f, 0.0f, 0.0f);
             if (gamma != 0.0f || mu != 0.0f) {
-                REPLAY_NEIGHBORS(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+                REPLAY_NEIGHBORS_BU(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
                     [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vel[k]}; },
                     [&](int k, const NbrXV& nb) {
                         if (k == s || !(nb.flags & kActive)) return;
@@ -1417,12 +1425,12 @@ struct PbfDeltaArgs {
 
 // compute_delta_position + apply_delta_position (+ apply_position_bounds after
 // the last iteration)  solver_pbf.py:305-363, 860-872
-template <int CAP, bool SCORR>
-__global__ __launch_bounds__(kPassBlock) void k_pbf_delta(PbfDeltaArgs a)
+template <int BLOCK, int CAP, int UNR, bool SCORR>
+__global__ __launch_bounds__(BLOCK) void k_pbf_delta(PbfDeltaArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
+    __shared__ int lds[BLOCK * CAP];
     PassCommon c = a.c;
-    const int s = pass_slot();
+    const int s = pass_slot_b<BLOCK>();
     if (s < c.n) {
         const int i = c.perm[s];
         const float4 xi4 = c.pos[s];
@@ -1436,7 +1444,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_delta(PbfDeltaArgs a)
             const float lam_i = a.sca[s].x;
             const float h = c.h;
             v3 dp = V(0.0f, 0.0f, 0.0f);
-            REPLAY_NEIGHBORS(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+            REPLAY_NEIGHBORS_BU(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
                 [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.sca[k].x}; },
                 [&](int k, const NbrXS& nb) {
                     if (k == s || !(nb.flags & kActive)) return;
@@ -1473,12 +1481,12 @@ struct PbfXsphArgs {
 };
 
 // apply_xsph_viscosity + copies + apply_bounds  solver_pbf.py:434-463, 919-954
-template <int CAP>
-__global__ __launch_bounds__(kPassBlock) void k_pbf_xsph(PbfXsphArgs a)
+template <int BLOCK, int CAP, int UNR>
+__global__ __launch_bounds__(BLOCK) void k_pbf_xsph(PbfXsphArgs a)
 {
-    __shared__ int lds[kPassBlock * CAP];
+    __shared__ int lds[BLOCK * CAP];
     PassCommon c = a.c;
-    const int s = pass_slot();
+    const int s = pass_slot_b<BLOCK>();
     if (s >= c.n) return;
     const int i = c.perm[s];
     v3 x = xyz(c.pos[s]);
@@ -1490,7 +1498,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pbf_xsph(PbfXsphArgs a)
     }
     const float h = c.h;
     v3 corr = V(0.0f, 0.0f, 0.0f);
-    REPLAY_NEIGHBORS(c, s, x.x, x.y, x.z, NoCount{}, phased(
+    REPLAY_NEIGHBORS_BU(c, s, x.x, x.y, x.z, NoCount{}, phased(
         [&](int k) { return NbrXV{c.sflags[k], c.pos[k], a.vel[k]}; },
         [&](int k, const NbrXV& nb) {
             if (k == s || !(nb.flags & kActive)) return;
@@ -1569,58 +1577,108 @@ __global__ __launch_bounds__(BLOCK) void k_separation(SepArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// launch variants of the tunable WCSPH passes: workgroup size and the depth of
-// the per-lane LDS survivor list (which bounds occupancy through LDS).  Every
-// variant visits the same candidates in the same order, so results are
+// launch variants of the tunable passes: workgroup size, the depth of the
+// per-lane LDS survivor list (which bounds occupancy through LDS), the number
+// of neighbours a list replay fetches together and the particles per lane.
+// Every variant visits the same candidates in the same order, so results are
 // bit-identical; only speed differs.  The selection is process-wide.
 // ---------------------------------------------------------------------------
 
 struct Variant {
     int block;
-    int cap;
+    int cap;      // LDS survivor-list entries per lane
+    int unroll;   // neighbours whose data a list replay fetches together
+    int ppt;      // particles per lane
 };
-// variant 0 is the default
-constexpr Variant kScanVariants[] = {{64, 48},  {64, 24},  {64, 32},  {64, 64},  {128, 24}, {128, 32},
-                                     {128, 16}, {256, 32}, {256, 24}, {256, 16}, {64, 40},  {64, 56}};
-constexpr Variant kReplayVariants[] = {{64, 16}, {128, 16}, {256, 16}, {512, 16}};
-constexpr int kNumScan = sizeof(kScanVariants) / sizeof(kScanVariants[0]);
-constexpr int kNumReplay = sizeof(kReplayVariants) / sizeof(kReplayVariants[0]);
-constexpr int kTunePasses = 4;   // FS_TUNE_WCSPH_DENSITY .. FS_TUNE_BATCH_FORCE
+// variant 0 of each table is the default
+constexpr Variant kScanVariants[] = {{64, 48, 4, 1},  {64, 24, 4, 1},  {64, 32, 4, 1},  {64, 64, 4, 1},
+                                     {128, 24, 4, 1}, {128, 32, 4, 1}, {128, 16, 4, 1}, {256, 32, 4, 1},
+                                     {256, 24, 4, 1}, {256, 16, 4, 1}, {64, 40, 4, 1},  {64, 56, 4, 1}};
+constexpr Variant kReplayVariants[] = {{64, 16, 4, 1},  {128, 16, 4, 1}, {256, 16, 4, 1}, {512, 16, 4, 1},
+                                       {64, 16, 8, 1},  {256, 16, 8, 1}, {64, 16, 4, 2},  {256, 16, 4, 2}};
+// the scanning (density/factor, density, lambda) and list-replaying passes of
+// DFSPH and PBF
+constexpr Variant kSolverScanVariants[] = {{64, 48, 4, 1}, {64, 40, 4, 1}, {64, 56, 4, 1}, {64, 32, 4, 1},
+                                           {128, 32, 4, 1}};
+constexpr Variant kSolverReplayVariants[] = {{256, 16, 4, 1}, {64, 16, 4, 1}, {128, 16, 4, 1},
+                                             {512, 16, 4, 1}, {256, 16, 8, 1}, {64, 16, 8, 1}};
+
+enum TunePass : int {
+    kTuneWcsphDensity = FS_TUNE_WCSPH_DENSITY,
+    kTuneWcsphForce = FS_TUNE_WCSPH_FORCE,
+    kTuneBatchDensity = FS_TUNE_BATCH_DENSITY,
+    kTuneBatchForce = FS_TUNE_BATCH_FORCE,
+    kTuneDfsphScan = FS_TUNE_DFSPH_SCAN,
+    kTuneDfsphReplay = FS_TUNE_DFSPH_REPLAY,
+    kTunePbfScan = FS_TUNE_PBF_SCAN,
+    kTunePbfReplay = FS_TUNE_PBF_REPLAY,
+};
+enum VariantTable : int { kTabScan = 0, kTabReplay = 1, kTabSolverScan = 2, kTabSolverReplay = 3 };
+constexpr int kTunePasses = 8;   // FS_TUNE_WCSPH_DENSITY .. FS_TUNE_PBF_REPLAY
+constexpr int kTuneTable[kTunePasses] = {kTabScan,       kTabReplay,       kTabScan,       kTabReplay,
+                                         kTabSolverScan, kTabSolverReplay, kTabSolverScan, kTabSolverReplay};
+
+constexpr int table_size(int t)
+{
+    return t == kTabScan     ? (int)(sizeof(kScanVariants) / sizeof(Variant))
+           : t == kTabReplay ? (int)(sizeof(kReplayVariants) / sizeof(Variant))
+           : t == kTabSolverScan ? (int)(sizeof(kSolverScanVariants) / sizeof(Variant))
+                                 : (int)(sizeof(kSolverReplayVariants) / sizeof(Variant));
+}
+constexpr Variant variant_at(int t, int i)
+{
+    return t == kTabScan     ? kScanVariants[i]
+           : t == kTabReplay ? kReplayVariants[i]
+           : t == kTabSolverScan ? kSolverScanVariants[i]
+                                 : kSolverReplayVariants[i];
+}
 
 static std::atomic<int> g_tune[kTunePasses];
 static std::mutex g_tune_mutex;
 
-static int tune_count(int pass) { return (pass == 0 || pass == 2) ? kNumScan : kNumReplay; }
+static int tune_count(int pass) { return table_size(kTuneTable[pass]); }
 static int tuned(int pass) { return g_tune[pass].load(std::memory_order_relaxed); }
-
-template <bool BATCH, int I = 0>
-static int32_t launch_density(int v, int n, hipStream_t s, const WcsphDensityArgs& a)
+static Variant tuned_variant(int pass)
 {
-    if constexpr (I < kNumScan) {
-        if (v == I) {
-            constexpr int B = kScanVariants[I].block, C = kScanVariants[I].cap;
-            FS_LAUNCH((k_wcsph_density<B, C, BATCH>), dim3(blocks_for(n, B)), dim3(B), 0, s, a);
-            return FS_OK;
-        }
-        return launch_density<BATCH, I + 1>(v, n, s, a);
+    const int t = kTuneTable[pass], v = tuned(pass);
+    switch (t) {
+    case kTabScan: return kScanVariants[v];
+    case kTabReplay: return kReplayVariants[v];
+    case kTabSolverScan: return kSolverScanVariants[v];
+    default: return kSolverReplayVariants[v];
     }
-    set_error("unknown launch variant %d", v);
-    return FS_ERR_ARG;
 }
 
-template <bool BATCH, int I = 0>
+// f.template operator()<V>() with V = variant v of table TABLE
+template <int TABLE, int I = 0, class F>
+static int32_t with_variant(int v, F&& f)
+{
+    if constexpr (I < table_size(TABLE)) {
+        if (v == I) return f.template operator()<variant_at(TABLE, I)>();
+        return with_variant<TABLE, I + 1>(v, f);
+    } else {
+        set_error("unknown launch variant %d", v);
+        return FS_ERR_ARG;
+    }
+}
+
+template <bool BATCH>
+static int32_t launch_density(int v, int n, hipStream_t s, const WcsphDensityArgs& a)
+{
+    return with_variant<kTabScan>(v, [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_wcsph_density<V.block, V.cap, BATCH>), dim3(blocks_for(n, V.block)), dim3(V.block), 0, s, a);
+        return FS_OK;
+    });
+}
+
+template <bool BATCH>
 static int32_t launch_force(int v, int n, hipStream_t s, const WcsphForceArgs& a)
 {
-    if constexpr (I < kNumReplay) {
-        if (v == I) {
-            constexpr int B = kReplayVariants[I].block, C = kReplayVariants[I].cap;
-            FS_LAUNCH((k_wcsph_force<B, C, BATCH>), dim3(blocks_for(n, B)), dim3(B), 0, s, a);
-            return FS_OK;
-        }
-        return launch_force<BATCH, I + 1>(v, n, s, a);
-    }
-    set_error("unknown launch variant %d", v);
-    return FS_ERR_ARG;
+    return with_variant<kTabReplay>(v, [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_wcsph_force<V.block, V.cap, BATCH, V.unroll, V.ppt>), dim3(blocks_for(n, V.block * V.ppt)),
+                  dim3(V.block), 0, s, a);
+        return FS_OK;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -1702,7 +1760,7 @@ size_t fs_work_bytes(int32_t capacity, int32_t list_capacity)
     if (capacity < 0 || list_capacity < 0) return 0;
     size_t cap = (size_t)(capacity > 0 ? capacity : 1);
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    size_t nblocks = (cap + kPassBlock - 1) / kPassBlock;
+    size_t nblocks = (cap + kMinPassBlock - 1) / kMinPassBlock;   // residual partials of the rate pass
     return al(cap * 16) * 4 + al(cap * 4) * 2 + al(cap * 12) * 2 + al(cap * 4 * (size_t)list_capacity) +
            al(cap * 4) + al(nblocks * 16 + 16) + al(kControlWords * 4) + al(8 * 4) + al(cap * 16);
 }
@@ -1724,7 +1782,7 @@ int32_t fs_work_init(fs_work* w, void* workspace, size_t bytes, int32_t capacity
         return FS_ERR_WORKSPACE;
     }
     size_t cap = (size_t)(capacity > 0 ? capacity : 1);
-    size_t nblocks = (cap + kPassBlock - 1) / kPassBlock;
+    size_t nblocks = (cap + kMinPassBlock - 1) / kMinPassBlock;
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     char* p = (char*)workspace;
     auto take = [&](size_t n) {
@@ -1872,9 +1930,20 @@ int32_t fs_tune_describe(int32_t pass, int32_t variant, int32_t* block, int32_t*
         set_error("fs_tune_describe: unknown pass %d / variant %d", pass, variant);
         return FS_ERR_ARG;
     }
-    const Variant& v = (pass == 0 || pass == 2) ? kScanVariants[variant] : kReplayVariants[variant];
+    const Variant v = variant_at(kTuneTable[pass], variant);
     if (block) *block = v.block;
     if (list_depth) *list_depth = v.cap;
+    return FS_OK;
+}
+
+int32_t fs_tune_describe_ex(int32_t pass, int32_t variant, int32_t* block, int32_t* list_depth, int32_t* unroll,
+                            int32_t* particles_per_thread)
+{
+    int32_t rc = fs_tune_describe(pass, variant, block, list_depth);
+    if (rc) return rc;
+    const Variant v = variant_at(kTuneTable[pass], variant);
+    if (unroll) *unroll = v.unroll;
+    if (particles_per_thread) *particles_per_thread = v.ppt;
     return FS_OK;
 }
 
@@ -2171,9 +2240,11 @@ static int32_t df_density_factor(const DfPlan& p, bool after_integrate)
     dfa.sca = p.sca;
     dfa.vel = p.vel;
     dfa.dt_device = p.prm->dt_device;
-    FS_LAUNCH((k_dfsph_density_factor<kScanBlock, kScanCap>), dim3(blocks_for(p.n, kScanBlock)), dim3(kScanBlock), 0,
-              p.s, dfa);
-    return FS_OK;
+    return with_variant<kTabSolverScan>(tuned(kTuneDfsphScan), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_dfsph_density_factor<V.block, V.cap>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s,
+                  dfa);
+        return FS_OK;
+    });
 }
 
 // acceleration_non_pressure + predict_velocity  (solver_dfsph.py:654-688)
@@ -2192,8 +2263,11 @@ static int32_t df_nonpressure_predict(const DfPlan& p)
     npa.accel = p.io->accel;
     npa.v_star = p.io->v_star;
     npa.vstar_s = p.vstar_s;
-    LAUNCH_PASS(k_dfsph_nonpressure<kReplayCap>, p.nb, p.s, npa);
-    return FS_OK;
+    return with_variant<kTabSolverReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_dfsph_nonpressure<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0,
+                  p.s, npa);
+        return FS_OK;
+    });
 }
 
 // apply_constant_density_correction (loop 0) or apply_divergence_free_correction
@@ -2219,8 +2293,11 @@ static int32_t df_correction(const DfPlan& p, int loop, bool warm)
         ca.corr_idx = loop == 0 ? kCorrRho : kCorrVel;
         ca.kappa_commit = loop == 0 ? p.io->kappa : p.io->kappa_v;
     }
-    LAUNCH_PASS(k_dfsph_correction<kReplayCap>, p.nb, p.s, ca);
-    return FS_OK;
+    return with_variant<kTabSolverReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_dfsph_correction<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0,
+                  p.s, ca);
+        return FS_OK;
+    });
 }
 
 // compute_density_change_rate + predict_density_star + compute_kappa_density
@@ -2241,8 +2318,11 @@ static int32_t df_rate_kappa(const DfPlan& p, int loop)
     ra.control = p.w->control;
     ra.density_loop = loop == 0 ? 1 : 0;
     ra.max_kappa = loop == 0 ? p.prm->max_kappa : p.prm->max_kappa_v;
-    LAUNCH_PASS(k_dfsph_rate<kReplayCap>, p.nb, p.s, ra);
-    return FS_OK;
+    return with_variant<kTabSolverReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_dfsph_rate<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s,
+                  ra);
+        return FS_OK;
+    });
 }
 
 // _density_residual / _divergence_residual and the loop's break rule
@@ -2251,7 +2331,7 @@ static int32_t df_residual(const DfPlan& p, int loop, int iteration)
 {
     ResidualArgs ra;
     ra.partials = p.w->partials;
-    ra.nb = p.nb;
+    ra.nb = blocks_for(p.n, tuned_variant(kTuneDfsphReplay).block);   // blocks of the rate pass
     ra.control = p.w->control;
     ra.stop_idx = loop == 0 ? kStopRho : kStopVel;
     ra.it_idx = loop == 0 ? kItRho : kItVel;
@@ -2546,8 +2626,10 @@ static int32_t pbf_density(const PbfPlan& p, bool final_pass)
     da.vel = p.vel;
     da.x_out = p.io->x_out;
     da.v_out = p.io->v_out;
-    FS_LAUNCH((k_pbf_density<kScanBlock, kScanCap>), dim3(blocks_for(p.n, kScanBlock)), dim3(kScanBlock), 0, p.s, da);
-    return FS_OK;
+    return with_variant<kTabSolverScan>(tuned(kTunePbfScan), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_pbf_density<V.block, V.cap>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s, da);
+        return FS_OK;
+    });
 }
 
 // acceleration_non_pressure + apply_non_pressure_prediction +
@@ -2566,8 +2648,11 @@ static int32_t pbf_nonpressure(const PbfPlan& p)
     na.x_pred = p.w->x_pred;
     na.v_pred = p.w->v_pred;
     na.pos_next = p.bufs[1];
-    LAUNCH_PASS(k_pbf_nonpressure<kReplayCap>, p.nb, p.s, na);
-    return FS_OK;
+    return with_variant<kTabSolverReplay>(tuned(kTunePbfReplay), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_pbf_nonpressure<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0,
+                  p.s, na);
+        return FS_OK;
+    });
 }
 
 // compute_pbf_lambda of iteration `it`  (solver_pbf.py:801-817)
@@ -2582,8 +2667,10 @@ static int32_t pbf_lambda(const PbfPlan& p, int it)
     la.rho = p.io->rho;
     la.neighbor_count = p.io->neighbor_count;
     la.sca = p.sca;
-    FS_LAUNCH((k_pbf_lambda<kScanBlock, kScanCap>), dim3(blocks_for(p.n, kScanBlock)), dim3(kScanBlock), 0, p.s, la);
-    return FS_OK;
+    return with_variant<kTabSolverScan>(tuned(kTunePbfScan), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_pbf_lambda<V.block, V.cap>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s, la);
+        return FS_OK;
+    });
 }
 
 // compute_delta_position + apply_delta_position (+ apply_position_bounds and
@@ -2606,8 +2693,16 @@ static int32_t pbf_delta(const PbfPlan& p, int it)
     pa.delta_p = p.io->delta_p;
     pa.x_pred = p.w->x_pred;
     pa.pos_next = p.bufs[cur ^ 1];
-    if (prm->scorr_k != 0.0f) LAUNCH_PASS((k_pbf_delta<kReplayCap, true>), p.nb, p.s, pa);
-    else LAUNCH_PASS((k_pbf_delta<kReplayCap, false>), p.nb, p.s, pa);
+    int32_t rc = with_variant<kTabSolverReplay>(tuned(kTunePbfReplay), [&]<Variant V>() -> int32_t {
+        if (prm->scorr_k != 0.0f)
+            FS_LAUNCH((k_pbf_delta<V.block, V.cap, V.unroll, true>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0,
+                      p.s, pa);
+        else
+            FS_LAUNCH((k_pbf_delta<V.block, V.cap, V.unroll, false>), dim3(blocks_for(p.n, V.block)), dim3(V.block),
+                      0, p.s, pa);
+        return FS_OK;
+    });
+    if (rc) return rc;
     if (prm->rebuild_grid_each_iteration) {
         PreOp rp;
         rp.x_in = p.w->x_pred;
@@ -2630,8 +2725,10 @@ static int32_t pbf_xsph_finalize(const PbfPlan& p)
     xa.bounds = p.bounds;
     xa.x_out = p.io->x_out;
     xa.v_out = p.io->v_out;
-    LAUNCH_PASS(k_pbf_xsph<kReplayCap>, p.nb, p.s, xa);
-    return FS_OK;
+    return with_variant<kTabSolverReplay>(tuned(kTunePbfReplay), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_pbf_xsph<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s, xa);
+        return FS_OK;
+    });
 }
 
 int32_t fs_pbf_step(fs_grid* g, fs_work* w, const fs_model* m, const fs_pbf_params* prm, const fs_pbf_io* io,

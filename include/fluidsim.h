@@ -61,18 +61,26 @@ int32_t fs_profile_report(char* buf, size_t len);
 
 /* ------------------------------------------------------------------------ */
 /* Launch-variant table of the tunable passes (no counterpart in the       */
-/* reference).  Each tunable pass has a few launch variants (workgroup size */
-/* x depth of the per-lane LDS survivor list) with bit-identical results;   */
-/* the selection is process-wide, guarded by a mutex, variant 0 by default. */
+/* reference).  Each tunable pass has a few launch variants (workgroup size, */
+/* depth of the per-lane LDS survivor list, neighbours a list replay fetches */
+/* together, particles per lane) with bit-identical results; the selection  */
+/* is process-wide, guarded by a mutex, variant 0 by default.               */
 /* ------------------------------------------------------------------------ */
 
 #define FS_TUNE_WCSPH_DENSITY 0   /* fs_wcsph_step density/EOS pass          */
 #define FS_TUNE_WCSPH_FORCE 1     /* fs_wcsph_step force/integrate pass      */
 #define FS_TUNE_BATCH_DENSITY 2   /* fs_batch_wcsph_step density/EOS pass    */
 #define FS_TUNE_BATCH_FORCE 3     /* fs_batch_wcsph_step force/integrate pass */
+#define FS_TUNE_DFSPH_SCAN 4      /* DFSPH density/factor passes             */
+#define FS_TUNE_DFSPH_REPLAY 5    /* DFSPH non-pressure, rate, correction    */
+#define FS_TUNE_PBF_SCAN 6        /* PBF density and lambda passes           */
+#define FS_TUNE_PBF_REPLAY 7      /* PBF non-pressure, delta, XSPH passes     */
 
 int32_t fs_tune_variant_count(int32_t pass);
 int32_t fs_tune_describe(int32_t pass, int32_t variant, int32_t* block, int32_t* list_depth);
+/* ... and the replay unroll and particles per lane of a variant */
+int32_t fs_tune_describe_ex(int32_t pass, int32_t variant, int32_t* block, int32_t* list_depth, int32_t* unroll,
+                            int32_t* particles_per_thread);
 int32_t fs_tune_select(int32_t pass, int32_t variant);
 int32_t fs_tune_selected(int32_t pass);
 

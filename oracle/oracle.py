@@ -411,14 +411,18 @@ def wcsph_adjoint_step(model: OModel, state_in: OState, cfg: WcsphConfig, dt: fl
     return xb, vb, gs, gv
 
 
-def wcsph_rollout(model: OModel, x0, v0, cfg: WcsphConfig, dt: float, steps: int, precision="f32"):
+def wcsph_rollout(model: OModel, x0, v0, cfg: WcsphConfig, dt: float, steps: int, precision="f32",
+                  densities: list | None = None):
     """``steps`` WCSPH substeps from (x0, v0); returns the list of states
-    [(x_0, v_0), ..., (x_steps, v_steps)]."""
+    [(x_0, v_0), ..., (x_steps, v_steps)].  With ``densities`` (a list), each
+    substep's density array is appended to it."""
     st = OState(np.array(x0), np.array(v0), {})
     states = [(st.x, st.v)]
     for _ in range(steps):
         st, _ = wcsph_step(model, st, cfg, dt, precision)
         states.append((st.x, st.v))
+        if densities is not None:
+            densities.append(st.sph["rho"])
     return states
 
 

@@ -40,6 +40,37 @@ def test_plain_variants_are_bit_identical():
         at.select(name, 0)
 
 
+@pytest.mark.parametrize("method", ["dfsph", "pbf"])
+def test_dfsph_pbf_variants_are_bit_identical(method):
+    """The scanning and list-replaying passes of DFSPH and PBF under every
+    launch variant (workgroup, LDS list depth, replay unroll): the same
+    substep, bit for bit, and the same DFSPH iteration counts."""
+    sim = DamBreakSimulation(make_scene(10000), method, device="cuda:0")
+    sim.step()
+    dt = sim.frame_dt / 4
+
+    def run():
+        s_out = sim.model.state()
+        sim.solver.step(sim.state_0, s_out, None, None, dt)
+        torch.cuda.synchronize()
+        out = [s_out.particle_q.numpy(), s_out.particle_qd.numpy(), s_out.sph.rho.numpy()]
+        if method == "dfsph":
+            out += [s_out.sph.factor.numpy(), s_out.sph.kappa_v.numpy(),
+                    np.array(list(sim.solver.last_iterations().values()))]
+        else:
+            out += [s_out.sph.pbf_lambda.numpy(), s_out.sph.neighbor_count.numpy()]
+        return out
+
+    ref = run()
+    for name in at.SOLVER_PASSES[method]:
+        for v, shape in enumerate(at.describe(name)):
+            print(f"{name} variant {v}: {shape}", flush=True)
+            at.select(name, v)
+            for a, b in zip(run(), ref):
+                np.testing.assert_array_equal(a, b)
+        at.select(name, 0)
+
+
 def test_batched_variants_are_bit_identical_and_autotune_restores_state():
     batch = BatchedWCSPH(make_batch_scenes(4, seed=2, count_range=(2000, 9000)), device="cuda:0")
     batch.begin_frame()

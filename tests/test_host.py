@@ -248,9 +248,15 @@ def test_tune_table_round_trip_and_cache(tmp_path, monkeypatch):
         "wcsph_density")[0]
     # cached selection is applied without timing anything
     monkeypatch.setenv("FLUIDSIM_TUNE_CACHE", str(tmp_path / "tune.json"))
-    monkeypatch.setattr(at.torch.cuda, "get_device_name", lambda *a: "dev")
+    monkeypatch.setattr(at, "device_arch", lambda: "dev")
     (tmp_path / "tune.json").write_text(json.dumps({"dev|w": {"wcsph_force": {"variant": 2, "block": 256,
                                                                              "list_depth": 16}}}))
     res = at.autotune(lambda: (_ for _ in ()).throw(AssertionError("must not run")), ("wcsph_force",), "w")
     assert res["wcsph_force"]["cached"] and at.selected("wcsph_force") == 2
     at.select("wcsph_force", 0)
+    # every solver's passes are tunable; variants carry unroll and particles per lane
+    assert set(p for ps in at.SOLVER_PASSES.values() for p in ps) <= set(at.PASSES)
+    shapes = {(v["unroll"], v["particles_per_thread"]) for v in at.describe("wcsph_force")}
+    assert {(4, 1), (8, 1), (4, 2)} <= shapes
+    assert {v["unroll"] for v in at.describe("dfsph_replay")} == {4, 8}
+    assert at.workload_key("dfsph", 401856) == "dfsph-n524288" and at.workload_key("pbf", 512) == "pbf-n512"
