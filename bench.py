@@ -223,6 +223,7 @@ def bench_solver(method, scene, steps, warmup, tune=True, settle_frames=0):
         tuned = tune_passes(lambda: sim.solver.step(sim.state_0, s_scratch, None, None, dt), SOLVER_PASSES[method])
         print(f"# {method} autotune: " + ", ".join(
             f"{k} -> block {v['block']} depth {v['list_depth']} unroll {v['unroll']} ppt {v['particles_per_thread']}"
+            f" lpp {v['lanes_per_particle']}"
             for k, v in tuned.items()), file=sys.stderr, flush=True)
     state = None
     if method == "wcsph":
@@ -258,6 +259,7 @@ def bench_solver(method, scene, steps, warmup, tune=True, settle_frames=0):
     if tuned:
         out["autotune"] = {k: {"block": v["block"], "list_depth": v["list_depth"], "unroll": v["unroll"],
                                "particles_per_thread": v["particles_per_thread"],
+                               "lanes_per_particle": v["lanes_per_particle"],
                                "us_per_step_by_variant": v["us_per_launch"]} for k, v in tuned.items()}
     if method == "dfsph":
         out["iterations_last_substep"] = iters
@@ -484,6 +486,7 @@ def batched(num_scenes, steps, warmup, chunk=10):
                        "speedup_vs_sequential_graph": seq_graph / graphed,
                        "autotune": {k: {"block": v["block"], "list_depth": v["list_depth"], "unroll": v["unroll"],
                                         "particles_per_thread": v["particles_per_thread"],
+                                        "lanes_per_particle": v["lanes_per_particle"],
                                         "us_per_launch_by_variant": v["us_per_launch"]} for k, v in tuned.items()}}}
     del graph, sims, batch
     torch.cuda.empty_cache()

@@ -146,9 +146,11 @@ SIGNATURES = {
     "fs_tune_variant_count": (c_int32, [c_int32]),
     "fs_tune_describe": (c_int32, [c_int32, c_int32, POINTER(c_int32), POINTER(c_int32)]),
     "fs_tune_describe_ex": (c_int32, [c_int32, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
-                                      POINTER(c_int32)]),
+                                      POINTER(c_int32), POINTER(c_int32)]),
     "fs_tune_select": (c_int32, [c_int32, c_int32]),
     "fs_tune_selected": (c_int32, [c_int32]),
+    "fs_check_fast_math": (c_int32, [c_int32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                     POINTER(ctypes.c_uint64), c_void_p]),
     "fs_grid_workspace_bytes": (c_size_t, [c_int32, c_int32, c_int32, c_int32]),
     "fs_grid_init": (c_int32, [POINTER(FsGrid), c_void_p, c_size_t, c_int32, c_int32, c_int32, c_int32, c_int32,
                                c_void_p]),

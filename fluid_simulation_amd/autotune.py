@@ -36,16 +36,17 @@ SOLVER_PASSES = {"wcsph": ("wcsph_density", "wcsph_force"), "dfsph": ("dfsph_sca
 
 
 def describe(pass_name: str) -> list[dict]:
-    """Every variant of a pass: workgroup size, LDS list depth, replay unroll
-    and particles per lane."""
+    """Every variant of a pass: workgroup size, LDS list depth, replay unroll,
+    particles per lane and lanes per particle."""
     lib = _lib.load()
     pid = PASSES[pass_name]
     out = []
     for v in range(lib.fs_tune_variant_count(pid)):
-        b, d, u, p = (ctypes.c_int32() for _ in range(4))
+        b, d, u, p, l = (ctypes.c_int32() for _ in range(5))
         _lib.check(lib.fs_tune_describe_ex(pid, v, ctypes.byref(b), ctypes.byref(d), ctypes.byref(u),
-                                           ctypes.byref(p)), "fs_tune_describe_ex")
-        out.append({"block": b.value, "list_depth": d.value, "unroll": u.value, "particles_per_thread": p.value})
+                                           ctypes.byref(p), ctypes.byref(l)), "fs_tune_describe_ex")
+        out.append({"block": b.value, "list_depth": d.value, "unroll": u.value, "particles_per_thread": p.value,
+                    "lanes_per_particle": l.value})
     return out
 
 

@@ -108,6 +108,137 @@ __device__ __forceinline__ v3 cubic_grad_r(v3 r, float r_norm, float h)
 }
 __device__ __forceinline__ v3 cubic_grad(v3 r, float h) { return cubic_grad_r(r, vlen(r), h); }
 
+// ---------------------------------------------------------------------------
+// Correctly rounded quotients and square roots without the range wrapping
+// ---------------------------------------------------------------------------
+// With correctly rounded float32 division, hipcc expands x / d into
+//   v_div_scale of d and of x, r = v_rcp(d) refined by one Newton step
+//   r += r (1 - d r), q = x r, two corrections q += r (x - d q) (the second
+//   one in v_div_fmas), and v_div_fixup;
+// and sqrtf(x) into v_sqrt_f32 followed by a one-ulp correction from two FMA
+// residuals, wrapped in a 2^32 scale for x < 2^-96 and a class test that
+// passes 0 and +inf through.  The wrapping only acts outside the ranges below:
+// for d normal with a normal reciprocal, x = 0 or |x| >= 2^-103, a normal
+// quotient and exponents of x and d less than 96 apart, v_div_scale returns
+// its operand and clears VCC (v_div_fmas is then a plain FMA) and v_div_fixup
+// only gives a zero quotient the sign of x; sqrt keeps x = 0 and x >= 2^-96
+// unscaled.  div_nr and sqrt_nr run the same core on operands their callers
+// keep inside those ranges, and div_nr takes the refined reciprocal as an
+// argument, so quotients with one divisor (a gradient's three components, a
+// pass's r / h) share it.  fs_check_fast_math compares them with the IEEE
+// forms on the GPU.
+__device__ __forceinline__ float rcp_nr(float d)
+{
+    const float r = __builtin_amdgcn_rcpf(d);
+    return __builtin_fmaf(__builtin_fmaf(-d, r, 1.0f), r, r);
+}
+// x / d for d > 0, r = rcp_nr(d)
+__device__ __forceinline__ float div_nr(float x, float d, float r)
+{
+    float q = x * r;
+    q = __builtin_fmaf(__builtin_fmaf(-d, q, x), r, q);
+    q = __builtin_fmaf(__builtin_fmaf(-d, q, x), r, q);
+    return __builtin_copysignf(q, x);
+}
+// x / d for x >= 0 (+0 stays +0 without the sign copy)
+__device__ __forceinline__ float div_nr_pos(float x, float d, float r)
+{
+    float q = x * r;
+    q = __builtin_fmaf(__builtin_fmaf(-d, q, x), r, q);
+    return __builtin_fmaf(__builtin_fmaf(-d, q, x), r, q);
+}
+// sqrtf(x) for x = 0 or 2^-96 <= x < inf
+__device__ __forceinline__ float sqrt_nr(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float lo = __uint_as_float(__float_as_uint(s) - 1u);
+    const float hi = __uint_as_float(__float_as_uint(s) + 1u);
+    const float t = __builtin_fmaf(-lo, s, x) <= 0.0f ? lo : s;
+    return __builtin_fmaf(-hi, s, x) > 0.0f ? hi : t;
+}
+
+// A pass's smoothing length with its refined reciprocal.  fast: h lies in
+// [2^-20, 2^20], where every quotient the kernel functions form from an
+// in-range separation (sep_in_range) is inside div_nr's ranges.
+struct Smooth {
+    float h;
+    float rh;
+    bool fast;
+};
+__device__ __forceinline__ Smooth smooth_of(float h)
+{
+    const bool fast = h >= 0x1p-20f && h <= 0x1p20f;
+    return Smooth{h, fast ? rcp_nr(h) : 1.0f, fast};
+}
+// Every component of r is 0 or of magnitude >= 2^-48: then r.r is 0 or
+// >= 2^-96 and every nonzero numerator of the gradient is >= 2^-48.
+// (bits << 1) - 1 drops the sign and maps +-0 to 0xffffffff.
+__device__ __forceinline__ bool sep_in_range(v3 r)
+{
+    const uint32_t kx = (__float_as_uint(r.x) << 1) - 1u;
+    const uint32_t ky = (__float_as_uint(r.y) << 1) - 1u;
+    const uint32_t kz = (__float_as_uint(r.z) << 1) - 1u;
+    return min(kx, min(ky, kz)) >= (0x27800000u << 1) - 1u;   // 0x27800000 = 2^-48
+}
+
+// get_cubic with q = r_norm / h from the shared reciprocal.  Equal to
+// cubic_w(r_norm, h) for every r_norm in [0, 2^100 h]: div_nr's ranges hold
+// for r_norm = 0 and r_norm >= 2^-103, and below that both quotients are
+// under 2^-83, where 6q^3 - 6q^2 + 1 rounds to 1 whatever q's low bits.
+__device__ __forceinline__ float cubic_w(float r_norm, const Smooth& s)
+{
+    if (!s.fast) return cubic_w(r_norm, s.h);
+    const float h = s.h;
+    float res = 0.0f;
+    float k = 8.0f / (kPi * (h * h * h));
+    float q = div_nr_pos(r_norm, h, s.rh);
+    if (q <= 1.0f) {
+        if (q <= 0.5f) {
+            float q2 = q * q;
+            float q3 = q2 * q;
+            res = k * (6.0f * q3 - 6.0f * q2 + 1.0f);
+        } else {
+            float t = 1.0f - q;
+            res = 2.0f * k * (t * t * t);
+        }
+    }
+    return res;
+}
+
+// |r| and cubic_grad_r(r, |r|, h): the shared-reciprocal forms when r and h
+// are in range, the IEEE forms otherwise.  Bit-identical to
+// (vlen(r), cubic_grad(r, h)).
+struct LenGrad {
+    float len;
+    v3 grad;
+};
+__device__ __forceinline__ LenGrad cubic_len_grad(v3 r, const Smooth& s)
+{
+    LenGrad o;
+    if (s.fast && sep_in_range(r)) {
+        const float h = s.h;
+        o.len = sqrt_nr(vdot(r, r));
+        o.grad = V(0.0f, 0.0f, 0.0f);
+        float k = 8.0f / (kPi * (h * h * h));
+        float q = div_nr_pos(o.len, h, s.rh);
+        if (o.len > 1.0e-5f && q <= 1.0f) {
+            const float D = o.len * h;
+            const float rD = rcp_nr(D);
+            v3 grad_q = V(div_nr(r.x, D, rD), div_nr(r.y, D, rD), div_nr(r.z, D, rD));
+            if (q <= 0.5f) {
+                o.grad = smul(6.0f * k * q * (3.0f * q - 2.0f), grad_q);
+            } else {
+                float q_term = 1.0f - q;
+                o.grad = smul(-6.0f * k * q_term * q_term, grad_q);
+            }
+        }
+    } else {
+        o.len = vlen(r);
+        o.grad = cubic_grad_r(r, o.len, s.h);
+    }
+    return o;
+}
+
 // H(r) u with H = d(cubic_grad)/dr on cubic_grad's branches (zero where it is
 // zero): cubic_grad = g(d) r, H u = g u + (g'(d)/d)(r.u) r, with
 //   q <= 0.5: g = 6k q(3q-2)/(d h),  g' = 18k/h^3
@@ -645,6 +776,104 @@ __device__ __forceinline__ int for_each_neighbor(const GridView& g, const float4
         NbrList scan = L;
         if (scan.mode == kListUse) scan.mode = kListNone;
         return scan_neighbors<BLOCK, CAP, false, BATCH>(g, pos, s, px, py, pz, r, cut2, lds, scan, count, body);
+    }
+}
+
+// Word W of v as held by lane W of this lane's group of LPP adjacent lanes
+// (DPP quad permutation: no LDS, usually folded into the consuming
+// instruction).  Every lane of the group must be active.
+template <int LPP, int W, class T>
+__device__ __forceinline__ T group_bcast(const T& v)
+{
+    if constexpr (LPP == 1) {
+        return v;
+    } else {
+        static_assert(LPP == 2 || LPP == 4, "lanes per particle");
+        static_assert(sizeof(T) % 4 == 0, "word-sized fields");
+        constexpr int ctrl = LPP == 4 ? W * 0x55 : (W == 0 ? 0xA0 : 0xF5);   // [W,W,W,W] / [0,0,2,2] / [1,1,3,3]
+        struct Words {
+            int w[sizeof(T) / 4];
+        };
+        Words in = __builtin_bit_cast(Words, v), out;
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(T) / 4); ++i) out.w[i] = __builtin_amdgcn_mov_dpp(in.w[i], ctrl, 0xF, 0xF, true);
+        return __builtin_bit_cast(T, out);
+    }
+}
+
+// List replay with LPP lanes per particle.  Lane q of the particle's group
+// evaluates term(k, data) for the entries e = q, q + LPP, ... (U of them per
+// trip, their indices fetched one trip ahead and their data together); the
+// group then hands the terms to fold() in entry order, lane by lane, so the
+// particle's sums are formed in the same order as by one lane walking the
+// list.  Every lane of the group folds every term and ends with the same sums.
+// The data loads are load_listed's: lists hold active neighbours only.
+template <int LPP, int U, class Load, class Term, class Fold>
+__device__ __forceinline__ void replay_split(const NbrList& L, int s, int m, Load& load, Term& term, Fold& fold)
+{
+    const int q = (int)(threadIdx.x & (LPP - 1));
+    const int* __restrict__ base = L.idx + s;
+    const ptrdiff_t n = L.n;
+    constexpr int kStep = LPP * U;   // entries per trip of the group
+    int k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int e = u * LPP + q;
+        k[u] = e < m ? base[(ptrdiff_t)e * n] : s;
+    }
+    for (int e0 = 0; e0 < m; e0 += kStep) {
+        int nk[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + kStep + u * LPP + q;
+            nk[u] = e < m ? base[(ptrdiff_t)e * n] : s;
+        }
+        decltype(load(0)) d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            d[u] = load(k[u]);
+            d[u].flags = kActive;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const auto t = term(k[u], d[u]);
+            if (e0 + u * LPP < m) fold(group_bcast<LPP, 0>(t));
+            if constexpr (LPP >= 2)
+                if (e0 + u * LPP + 1 < m) fold(group_bcast<LPP, 1>(t));
+            if constexpr (LPP == 4) {
+                if (e0 + u * LPP + 2 < m) fold(group_bcast<LPP, 2>(t));
+                if (e0 + u * LPP + 3 < m) fold(group_bcast<LPP, 3>(t));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) k[u] = nk[u];
+    }
+}
+
+// Neighbour loop of a list-replaying pass split into term (per neighbour,
+// order-free) and fold (accumulation in visit order), run by LPP lanes per
+// particle.  Without a usable list (no list, or an overflowed one) the
+// group's first lane scans alone with body fold(term(k, data)); then only
+// that lane's sums are the particle's.
+template <int BLOCK, int CAP, int UNR, int LPP, class Load, class Term, class Fold>
+__device__ __forceinline__ void for_each_neighbor_split(const GridView& g, const float4* __restrict__ pos, int s,
+                                                        float px, float py, float pz, float r, float cut2,
+                                                        int* __restrict__ lds, const NbrList& L, Load load, Term term,
+                                                        Fold fold)
+{
+    if (L.mode == kListUse) {
+        const int m = L.cnt[s];
+        if (m <= L.cap) {
+            replay_split<LPP, UNR>(L, s, m, load, term, fold);
+            return;
+        }
+    }
+    if ((threadIdx.x & (LPP - 1)) == 0) {
+        NbrList none = L;
+        none.mode = kListNone;
+        scan_neighbors<BLOCK, CAP, true, kBatch>(
+            g, pos, s, px, py, pz, r, cut2, lds, none, NoCount{},
+            phased(load, [&](int k, const decltype(load(0))& d) { fold(term(k, d)); }));
     }
 }
 

@@ -95,6 +95,9 @@ struct AdjNbr {   // reverse force pass: flags, x + mass, abar + c, v + m/rho, p
 __device__ __forceinline__ int pass_slot() { return xcd_tile() * kPassBlock + threadIdx.x; }
 template <int BLOCK>
 __device__ __forceinline__ int pass_slot_b() { return xcd_tile() * BLOCK + threadIdx.x; }
+// slot of a lane's particle with LPP adjacent lanes per particle
+template <int BLOCK, int LPP>
+__device__ __forceinline__ int pass_slot_g() { return xcd_tile() * (BLOCK / LPP) + (int)(threadIdx.x / LPP); }
 
 // Per-lane view of a batch scene: its bucket table, cell width and radius.
 __device__ __forceinline__ void scene_view(PassCommon& c, const fs_scene& sc)
@@ -433,14 +436,14 @@ __device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, co
         if (c.L.mode == kListBuild) c.L.cnt[s] = 0;
         return;
     }
-    const float h = c.h;
+    const Smooth sm = smooth_of(c.h);
     float rho_temp = 0.0f;
     SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
         [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.wvol[k]}; },
         [&](int k, const NbrXS& nb) {
             if (!(nb.flags & kActive)) return;
             v3 dist = vsub(xyz(xi), xyz(nb.x));
-            rho_temp += nb.s * cubic_w(vlen(dist), h);
+            rho_temp += nb.s * cubic_w(vlen(dist), sm);
         }));
     const float rho_i = rho_temp;
     float p = a.stiffness[i] * (fpow_(rho_i / fmax_(a.rest_density[i], 1.0e-6f), a.exponent[i]) - 1.0f);
@@ -542,6 +545,7 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
     const float dp_i = si.z, m_i = xi4.w;
     const float mu = a.viscosity[i], gamma = a.surface_tension[i];
     const float h = c.h;
+    const Smooth sm = smooth_of(h);
     const float visc_den0 = 0.01f * (h * h);
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
@@ -551,11 +555,12 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
             if (k == s || !(nb.flags & kActive)) return;
             const float4 xj = nb.x;
             const v3 d = vsub(xi, xyz(xj));
-            const float distance = vlen(d);
+            const LenGrad lg = cubic_len_grad(d, sm);
+            const float distance = lg.len;
             const float mass_nei = xj.w;
-            const v3 grad = cubic_grad_r(d, distance, h);
+            const v3 grad = lg.grad;
             if (gamma != 0.0f)
-                acc = vsub(acc, muls(smul(gamma / m_i * mass_nei, d), cubic_w(fmax_(distance, pr), h)));
+                acc = vsub(acc, muls(smul(gamma / m_i * mass_nei, d), cubic_w(fmax_(distance, pr), sm)));
             if (mu != 0.0f) {
                 const float4 vj = nb.v;
                 float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
@@ -864,6 +869,7 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
     }
     const v3 xi = xyz(xi4);
     const float h = a.c.h;
+    const Smooth sm = smooth_of(h);
     float rho_temp = 0.0f;
     v3 sum_grad = V(0.0f, 0.0f, 0.0f);
     float sum_grad_sq = 0.0f;
@@ -872,9 +878,9 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
                            if (!(nb.flags & kActive)) return;
                            const float4 xj = nb.x;
                            v3 r = vsub(xi, xyz(xj));
-                           const float rn = vlen(r);
-                           rho_temp += nb.s * cubic_w(rn, h);
-                           v3 grad = smul(xj.w, cubic_grad_r(r, rn, h));
+                           const LenGrad lg = cubic_len_grad(r, sm);
+                           rho_temp += nb.s * cubic_w(lg.len, sm);
+                           v3 grad = smul(xj.w, lg.grad);
                            sum_grad = vadd(sum_grad, grad);
                            sum_grad_sq += vdot(grad, grad);
                        });
@@ -949,6 +955,7 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_nonpressure(DfNonPressureArgs a
     const v3 vi = xyz(a.vel[s]);
     const float m_i = xi4.w, mu = a.viscosity[i], gamma = a.surface_tension[i];
     const float h = a.c.h, pr = a.particle_radius;
+    const Smooth sm = smooth_of(h);
     const float visc_den0 = 0.01f * (h * h);
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
@@ -959,14 +966,15 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_nonpressure(DfNonPressureArgs a
                 if (k == s || !(nb.flags & kActive)) return;
                 const float4 xj = nb.x;
                 const v3 d = vsub(xi, xyz(xj));
-                const float dist = vlen(d);
+                const LenGrad lg = cubic_len_grad(d, sm);
+                const float dist = lg.len;
                 if (gamma != 0.0f)
-                    acc = vsub(acc, muls(smul(gamma / m_i * xj.w, d), cubic_w(fmax_(dist, pr), h)));
+                    acc = vsub(acc, muls(smul(gamma / m_i * xj.w, d), cubic_w(fmax_(dist, pr), sm)));
                 if (mu != 0.0f) {
                     const float4 vj = nb.v;
                     float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
                     float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
-                    acc = vadd(acc, smul(sv, cubic_grad_r(d, dist, h)));
+                    acc = vadd(acc, smul(sv, lg.grad));
                 }
             }));
     }
@@ -997,7 +1005,7 @@ struct DfRateArgs {
 // compute_density_change_rate (+ predict_density_star, compute_kappa_density
 // or compute_kappa_divergence, and the host residual)  solver_dfsph.py:245-306,
 // 356-372, 579-596
-template <int BLOCK, int CAP, int UNR>
+template <int BLOCK, int CAP, int UNR, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_dfsph_rate(DfRateArgs a)
 {
     __shared__ int lds[BLOCK * CAP];
@@ -1005,55 +1013,65 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_rate(DfRateArgs a)
     if (a.control[stop_idx]) return;
     float dt;
     if (!substep_dt(a.dt_device, a.dt, dt)) return;
-    const int s = pass_slot_b<BLOCK>();
+    const int s = pass_slot_g<BLOCK, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;   // the lane that writes the particle's results
     double res_v = 0.0, res_c = 0.0;
     if (s < a.c.n) {
         const int i = a.c.perm[s];
         const int fl = a.c.sflags[s];
         float4 si = a.sca[s];
         if (!(fl & kActive)) {
-            a.d_rho_dt[i] = 0.0f;
-            if (a.density_loop) a.rho_star[i] = 0.0f;
-            si.z = 0.0f;
-            si.w = 0.0f / fmax_(si.x, 1.0e-6f);
-            a.sca[s] = si;
+            if (lead) {
+                a.d_rho_dt[i] = 0.0f;
+                if (a.density_loop) a.rho_star[i] = 0.0f;
+                si.z = 0.0f;
+                si.w = 0.0f / fmax_(si.x, 1.0e-6f);
+                a.sca[s] = si;
+            }
         } else {
             const v3 xi = xyz(a.c.pos[s]);
             const v3 vi = xyz(a.vstar_s[s]);
-            const float h = a.c.h;
+            const Smooth sm = smooth_of(a.c.h);
             float div_val = 0.0f;
-            REPLAY_NEIGHBORS_BU(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+            // a skipped (inactive) neighbour contributes +0: div_val starts at
+            // +0 and a round-to-nearest sum from +0 never becomes -0, so adding
+            // +0 leaves it unchanged
+            for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
+                a.c.g, a.c.pos, s, xi.x, xi.y, xi.z, a.c.h, a.c.cut2, lds, a.c.L,
                 [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vstar_s[k]}; },
-                [&](int k, const NbrXV& nb) {
-                    if (!(nb.flags & kActive)) return;
+                [&](int k, const NbrXV& nb) -> float {
+                    if (!(nb.flags & kActive)) return 0.0f;
                     const float4 xj = nb.x;
-                    v3 grad_w = cubic_grad(vsub(xi, xyz(xj)), h);
-                    div_val += xj.w * vdot(vsub(vi, xyz(nb.v)), grad_w);
-                }));
-            a.d_rho_dt[i] = div_val;
-            const bool dyn = (fl & kDynamic) != 0;
-            float kap = 0.0f;
-            if (a.density_loop) {
-                const float rs = si.x + dt * div_val;
-                a.rho_star[i] = rs;
-                const float diff = rs - a.rest_density[i];
-                const float err = fmax_(diff, 0.0f);
-                if (dyn) {
-                    res_v = (double)np_max0(diff);
-                    res_c = 1.0;
-                    kap = fmin_((err / (dt * dt)) * si.y, a.max_kappa);
+                    v3 grad_w = cubic_len_grad(vsub(xi, xyz(xj)), sm).grad;
+                    return xj.w * vdot(vsub(vi, xyz(nb.v)), grad_w);
+                },
+                [&](float t) { div_val += t; });
+            if (lead) {
+                a.d_rho_dt[i] = div_val;
+                const bool dyn = (fl & kDynamic) != 0;
+                float kap = 0.0f;
+                if (a.density_loop) {
+                    const float rs = si.x + dt * div_val;
+                    a.rho_star[i] = rs;
+                    const float diff = rs - a.rest_density[i];
+                    const float err = fmax_(diff, 0.0f);
+                    if (dyn) {
+                        res_v = (double)np_max0(diff);
+                        res_c = 1.0;
+                        kap = fmin_((err / (dt * dt)) * si.y, a.max_kappa);
+                    }
+                } else {
+                    const float dc = fmax_(div_val, 0.0f);
+                    if (dyn) {
+                        res_v = (double)np_max0(div_val);
+                        res_c = 1.0;
+                        kap = fmin_((1.0f / dt) * dc * si.x * si.y, a.max_kappa);
+                    }
                 }
-            } else {
-                const float dc = fmax_(div_val, 0.0f);
-                if (dyn) {
-                    res_v = (double)np_max0(div_val);
-                    res_c = 1.0;
-                    kap = fmin_((1.0f / dt) * dc * si.x * si.y, a.max_kappa);
-                }
+                si.z = kap;
+                si.w = kap / fmax_(si.x, 1.0e-6f);
+                a.sca[s] = si;
             }
-            si.z = kap;
-            si.w = kap / fmax_(si.x, 1.0e-6f);
-            a.sca[s] = si;
         }
     }
     residual_partials<BLOCK>(res_v, res_c, a.partials);
@@ -1074,32 +1092,37 @@ struct DfCorrArgs {
 
 // apply_constant_density_correction / apply_divergence_free_correction
 // solver_dfsph.py:309-337, 375-403 (and the kappa write of :290-306, 356-372)
-template <int BLOCK, int CAP, int UNR>
+template <int BLOCK, int CAP, int UNR, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_dfsph_correction(DfCorrArgs a)
 {
     __shared__ int lds[BLOCK * CAP];
     if (a.stop_idx >= 0 && a.control[a.stop_idx]) return;
     float dt;
     if (!substep_dt(a.dt_device, a.dt, dt)) return;
-    const int s = pass_slot_b<BLOCK>();
+    const int s = pass_slot_g<BLOCK, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     if (blockIdx.x == 0 && threadIdx.x == 0 && a.corr_idx >= 0) a.control[a.corr_idx] += 1;
     if (s >= a.c.n) return;
     const int i = a.c.perm[s];
     const float4 si = a.sca[s];
-    if (a.kappa_commit) a.kappa_commit[i] = si.z;
+    if (a.kappa_commit && lead) a.kappa_commit[i] = si.z;
     if (!(a.c.sflags[s] & kDynamic)) return;
     const v3 xi = xyz(a.c.pos[s]);
-    const float h = a.c.h;
+    const Smooth sm = smooth_of(a.c.h);
     const float kri = si.w;
     v3 dv = V(0.0f, 0.0f, 0.0f);
-    REPLAY_NEIGHBORS_BU(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+    // skipped neighbours contribute +0 (see k_dfsph_rate)
+    for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
+        a.c.g, a.c.pos, s, xi.x, xi.y, xi.z, a.c.h, a.c.cut2, lds, a.c.L,
         [&](int k) { return NbrXS{a.c.sflags[k], a.c.pos[k], a.sca[k].w}; },
-        [&](int k, const NbrXS& nb) {
-            if (!(nb.flags & kActive)) return;
+        [&](int k, const NbrXS& nb) -> v3 {
+            if (!(nb.flags & kActive)) return V(0.0f, 0.0f, 0.0f);
             const float4 xj = nb.x;
             float coeff = xj.w * (kri + nb.s);
-            dv = vadd(dv, smul(coeff, cubic_grad(vsub(xi, xyz(xj)), h)));
-        }));
+            return smul(coeff, cubic_len_grad(vsub(xi, xyz(xj)), sm).grad);
+        },
+        [&](const v3& t) { dv = vadd(dv, t); });
+    if (!lead) return;
     const float4 vs4 = a.vstar_s[s];
     const v3 vs = vsub(xyz(vs4), smul(dt, dv));
     st3(a.v_star, i, vs);
@@ -1586,9 +1609,10 @@ __global__ __launch_bounds__(BLOCK) void k_separation(SepArgs a)
 
 struct Variant {
     int block;
-    int cap;      // LDS survivor-list entries per lane
-    int unroll;   // neighbours whose data a list replay fetches together
-    int ppt;      // particles per lane
+    int cap;       // LDS survivor-list entries per lane
+    int unroll;    // neighbours whose data a list replay fetches together (per lane)
+    int ppt;       // particles per lane
+    int lpp = 1;   // lanes per particle (block / lpp >= 64 particles per workgroup)
 };
 // variant 0 of each table is the default
 constexpr Variant kScanVariants[] = {{64, 48, 4, 1},  {64, 24, 4, 1},  {64, 32, 4, 1},  {64, 64, 4, 1},
@@ -1602,6 +1626,12 @@ constexpr Variant kSolverScanVariants[] = {{64, 48, 4, 1}, {64, 40, 4, 1}, {64, 
                                            {128, 32, 4, 1}};
 constexpr Variant kSolverReplayVariants[] = {{256, 16, 4, 1}, {64, 16, 4, 1}, {128, 16, 4, 1},
                                              {512, 16, 4, 1}, {256, 16, 8, 1}, {64, 16, 8, 1}};
+// DFSPH's list-replaying passes: the rate and correction passes also run with
+// 2 or 4 lanes per particle (the non-pressure pass with one)
+constexpr Variant kDfsphReplayVariants[] = {
+    {256, 16, 4, 1, 1}, {64, 16, 4, 1, 1},  {128, 16, 4, 1, 1}, {512, 16, 4, 1, 1},
+    {256, 16, 8, 1, 1}, {64, 16, 8, 1, 1},  {256, 16, 4, 1, 4}, {256, 16, 2, 1, 4},
+    {512, 16, 4, 1, 4}, {512, 16, 2, 1, 4}, {256, 16, 4, 1, 2}, {128, 16, 4, 1, 2}};
 
 enum TunePass : int {
     kTuneWcsphDensity = FS_TUNE_WCSPH_DENSITY,
@@ -1613,24 +1643,26 @@ enum TunePass : int {
     kTunePbfScan = FS_TUNE_PBF_SCAN,
     kTunePbfReplay = FS_TUNE_PBF_REPLAY,
 };
-enum VariantTable : int { kTabScan = 0, kTabReplay = 1, kTabSolverScan = 2, kTabSolverReplay = 3 };
+enum VariantTable : int { kTabScan = 0, kTabReplay = 1, kTabSolverScan = 2, kTabSolverReplay = 3, kTabDfsphReplay = 4 };
 constexpr int kTunePasses = 8;   // FS_TUNE_WCSPH_DENSITY .. FS_TUNE_PBF_REPLAY
-constexpr int kTuneTable[kTunePasses] = {kTabScan,       kTabReplay,       kTabScan,       kTabReplay,
-                                         kTabSolverScan, kTabSolverReplay, kTabSolverScan, kTabSolverReplay};
+constexpr int kTuneTable[kTunePasses] = {kTabScan,       kTabReplay,      kTabScan,       kTabReplay,
+                                         kTabSolverScan, kTabDfsphReplay, kTabSolverScan, kTabSolverReplay};
 
 constexpr int table_size(int t)
 {
     return t == kTabScan     ? (int)(sizeof(kScanVariants) / sizeof(Variant))
            : t == kTabReplay ? (int)(sizeof(kReplayVariants) / sizeof(Variant))
            : t == kTabSolverScan ? (int)(sizeof(kSolverScanVariants) / sizeof(Variant))
-                                 : (int)(sizeof(kSolverReplayVariants) / sizeof(Variant));
+           : t == kTabSolverReplay ? (int)(sizeof(kSolverReplayVariants) / sizeof(Variant))
+                                   : (int)(sizeof(kDfsphReplayVariants) / sizeof(Variant));
 }
 constexpr Variant variant_at(int t, int i)
 {
     return t == kTabScan     ? kScanVariants[i]
            : t == kTabReplay ? kReplayVariants[i]
            : t == kTabSolverScan ? kSolverScanVariants[i]
-                                 : kSolverReplayVariants[i];
+           : t == kTabSolverReplay ? kSolverReplayVariants[i]
+                                   : kDfsphReplayVariants[i];
 }
 
 static std::atomic<int> g_tune[kTunePasses];
@@ -1645,7 +1677,8 @@ static Variant tuned_variant(int pass)
     case kTabScan: return kScanVariants[v];
     case kTabReplay: return kReplayVariants[v];
     case kTabSolverScan: return kSolverScanVariants[v];
-    default: return kSolverReplayVariants[v];
+    case kTabSolverReplay: return kSolverReplayVariants[v];
+    default: return kDfsphReplayVariants[v];
     }
 }
 
@@ -1937,13 +1970,14 @@ int32_t fs_tune_describe(int32_t pass, int32_t variant, int32_t* block, int32_t*
 }
 
 int32_t fs_tune_describe_ex(int32_t pass, int32_t variant, int32_t* block, int32_t* list_depth, int32_t* unroll,
-                            int32_t* particles_per_thread)
+                            int32_t* particles_per_thread, int32_t* lanes_per_particle)
 {
     int32_t rc = fs_tune_describe(pass, variant, block, list_depth);
     if (rc) return rc;
     const Variant v = variant_at(kTuneTable[pass], variant);
     if (unroll) *unroll = v.unroll;
     if (particles_per_thread) *particles_per_thread = v.ppt;
+    if (lanes_per_particle) *lanes_per_particle = v.lpp;
     return FS_OK;
 }
 
@@ -2263,7 +2297,7 @@ static int32_t df_nonpressure_predict(const DfPlan& p)
     npa.accel = p.io->accel;
     npa.v_star = p.io->v_star;
     npa.vstar_s = p.vstar_s;
-    return with_variant<kTabSolverReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
+    return with_variant<kTabDfsphReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
         FS_LAUNCH((k_dfsph_nonpressure<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0,
                   p.s, npa);
         return FS_OK;
@@ -2293,9 +2327,9 @@ static int32_t df_correction(const DfPlan& p, int loop, bool warm)
         ca.corr_idx = loop == 0 ? kCorrRho : kCorrVel;
         ca.kappa_commit = loop == 0 ? p.io->kappa : p.io->kappa_v;
     }
-    return with_variant<kTabSolverReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_dfsph_correction<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0,
-                  p.s, ca);
+    return with_variant<kTabDfsphReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_dfsph_correction<V.block, V.cap, V.unroll, V.lpp>), dim3(blocks_for(p.n, V.block / V.lpp)),
+                  dim3(V.block), 0, p.s, ca);
         return FS_OK;
     });
 }
@@ -2318,9 +2352,9 @@ static int32_t df_rate_kappa(const DfPlan& p, int loop)
     ra.control = p.w->control;
     ra.density_loop = loop == 0 ? 1 : 0;
     ra.max_kappa = loop == 0 ? p.prm->max_kappa : p.prm->max_kappa_v;
-    return with_variant<kTabSolverReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_dfsph_rate<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s,
-                  ra);
+    return with_variant<kTabDfsphReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
+        FS_LAUNCH((k_dfsph_rate<V.block, V.cap, V.unroll, V.lpp>), dim3(blocks_for(p.n, V.block / V.lpp)),
+                  dim3(V.block), 0, p.s, ra);
         return FS_OK;
     });
 }
@@ -2331,7 +2365,8 @@ static int32_t df_residual(const DfPlan& p, int loop, int iteration)
 {
     ResidualArgs ra;
     ra.partials = p.w->partials;
-    ra.nb = blocks_for(p.n, tuned_variant(kTuneDfsphReplay).block);   // blocks of the rate pass
+    const Variant rv = tuned_variant(kTuneDfsphReplay);
+    ra.nb = blocks_for(p.n, rv.block / rv.lpp);   // blocks of the rate pass
     ra.control = p.w->control;
     ra.stop_idx = loop == 0 ? kStopRho : kStopVel;
     ra.it_idx = loop == 0 ? kItRho : kItVel;

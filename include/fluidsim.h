@@ -63,8 +63,9 @@ int32_t fs_profile_report(char* buf, size_t len);
 /* Launch-variant table of the tunable passes (no counterpart in the       */
 /* reference).  Each tunable pass has a few launch variants (workgroup size, */
 /* depth of the per-lane LDS survivor list, neighbours a list replay fetches */
-/* together, particles per lane) with bit-identical results; the selection  */
-/* is process-wide, guarded by a mutex, variant 0 by default.               */
+/* together, particles per lane, lanes per particle) with bit-identical     */
+/* results; the selection is process-wide, guarded by a mutex, variant 0 by */
+/* default.                                                                 */
 /* ------------------------------------------------------------------------ */
 
 #define FS_TUNE_WCSPH_DENSITY 0   /* fs_wcsph_step density/EOS pass          */
@@ -78,11 +79,24 @@ int32_t fs_profile_report(char* buf, size_t len);
 
 int32_t fs_tune_variant_count(int32_t pass);
 int32_t fs_tune_describe(int32_t pass, int32_t variant, int32_t* block, int32_t* list_depth);
-/* ... and the replay unroll and particles per lane of a variant */
+/* ... and the replay unroll (neighbours a lane fetches together), particles
+ * per lane and lanes per particle of a variant (any pointer may be NULL) */
 int32_t fs_tune_describe_ex(int32_t pass, int32_t variant, int32_t* block, int32_t* list_depth, int32_t* unroll,
-                            int32_t* particles_per_thread);
+                            int32_t* particles_per_thread, int32_t* lanes_per_particle);
 int32_t fs_tune_select(int32_t pass, int32_t variant);
 int32_t fs_tune_selected(int32_t pass);
+
+/* Diagnostic (no counterpart in the reference): compares, on the device,
+ * the shared-reciprocal float32 quotient and square-root forms the
+ * neighbour passes use on in-range operands with the correctly rounded '/'
+ * and sqrtf.  mode 0: sqrt of x = bits(first + t); 1: random in-range (x, d)
+ * pairs; 2: x = bits(first + t) over d = bits(seed); 3: random separations
+ * and smoothing lengths through the cubic kernel's |r|, gradient and value;
+ * t runs over [0, count).  result (host uint64[2]) = number of bitwise
+ * mismatches and the first mismatching t (UINT64_MAX if none).
+ * Synchronises the stream. */
+int32_t fs_check_fast_math(int32_t mode, uint64_t first, uint64_t count, uint32_t seed, uint64_t* result,
+                           fs_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Hash grid                                                                */

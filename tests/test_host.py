@@ -258,5 +258,7 @@ def test_tune_table_round_trip_and_cache(tmp_path, monkeypatch):
     assert set(p for ps in at.SOLVER_PASSES.values() for p in ps) <= set(at.PASSES)
     shapes = {(v["unroll"], v["particles_per_thread"]) for v in at.describe("wcsph_force")}
     assert {(4, 1), (8, 1), (4, 2)} <= shapes
-    assert {v["unroll"] for v in at.describe("dfsph_replay")} == {4, 8}
+    assert {v["unroll"] for v in at.describe("dfsph_replay")} == {2, 4, 8}
+    assert {v["lanes_per_particle"] for v in at.describe("dfsph_replay")} == {1, 2, 4}
+    assert all(v["block"] // v["lanes_per_particle"] >= 64 for p in at.PASSES for v in at.describe(p))
     assert at.workload_key("dfsph", 401856) == "dfsph-n524288" and at.workload_key("pbf", 512) == "pbf-n512"
