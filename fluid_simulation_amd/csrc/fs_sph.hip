@@ -527,15 +527,81 @@ struct WcsphForceArgs {
 // p/max(rho^2,1e-6), m/max(rho,1e-6) precomputed -- the same float operations
 // on the same operands.  A zero cohesion (gamma) or viscosity (mu) coefficient
 // contributes an exact zero, so its term is skipped.
-template <int BLOCK, int CAP, int UNR>
+// A neighbour's three updates of the particle's acceleration, applied in
+// this order: cohesion (COH only: surface tension != 0), viscosity, pressure.
+// With mu = 0 the viscosity term is an exact zero, and adding it leaves the
+// sum unchanged (a round-to-nearest sum from +0 never becomes -0).
+template <bool COH>
+struct ForceTerms {
+    v3 visc, pres;
+};
+template <>
+struct ForceTerms<true> {
+    v3 coh, visc, pres;
+};
+
+template <int BLOCK, int CAP, int UNR, int LPP, bool COH>
+__device__ __forceinline__ v3 wcsph_force_sum(const WcsphForceArgs& a, const PassCommon& c, float pr, int s,
+                                              v3 xi, v3 vi, float dp_i, float m_i, float mu, float gamma, int* lds)
+{
+    const float h = c.h;
+    const Smooth sm = smooth_of(h);
+    const float visc_den0 = 0.01f * (h * h);
+    const float mu10 = 10.0f * mu;
+    v3 acc = V(0.0f, 0.0f, 0.0f);
+    for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
+        c.g, c.pos, s, xi.x, xi.y, xi.z, c.h, c.cut2, lds, c.L,
+        [&](int k) { return NbrXVS{c.sflags[k], c.pos[k], a.vel[k], a.sca[k].z}; },
+        [&](int k, const NbrXVS& nb) -> ForceTerms<COH> {
+            ForceTerms<COH> t;
+            if constexpr (COH) t.coh = V(0.0f, 0.0f, 0.0f);
+            t.visc = V(0.0f, 0.0f, 0.0f);
+            t.pres = V(0.0f, 0.0f, 0.0f);
+            if (k == s || !(nb.flags & kActive)) return t;
+            const float4 xj = nb.x;
+            const v3 d = vsub(xi, xyz(xj));
+            const LenGrad lg = cubic_len_grad(d, sm);
+            const float distance = lg.len;
+            const float mass_nei = xj.w;
+            const v3 grad = lg.grad;
+            if constexpr (COH) t.coh = muls(smul(gamma / m_i * mass_nei, d), cubic_w(fmax_(distance, pr), sm));
+            if (mu != 0.0f) {
+                const float4 vj = nb.v;
+                float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
+                float sv = mu10 * vj.w * v_current_nei / (distance * distance + visc_den0);
+                t.visc = smul(sv, grad);
+            }
+            t.pres = smul(-mass_nei * (dp_i + nb.s), grad);
+            return t;
+        },
+        [&](const ForceTerms<COH>& t) {
+            if constexpr (COH) acc = vsub(acc, t.coh);
+            acc = vadd(acc, t.visc);
+            acc = vadd(acc, t.pres);
+        });
+    return acc;
+}
+
+// acceleration_wcsph + integrate_velocity + integrate_positions + apply_bounds
+// solver_wcsph.py:161-305.  Per neighbour the operations of
+// cal_acc_with_non_pressure and cal_acc_with_pressure (:63-106) are evaluated
+// with the kernel gradient computed once and the per-particle quotients
+// p/max(rho^2,1e-6), m/max(rho,1e-6) precomputed -- the same float operations
+// on the same operands.  A zero cohesion (gamma) coefficient contributes an
+// exact zero, so its term is skipped.  LPP lanes per particle share the
+// neighbour terms; the group's first lane writes.
+template <int BLOCK, int CAP, int UNR, int LPP>
 __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const PassCommon& c, float pr, float dt,
                                                  float dmp, float sx, float sy, float sz, int s, int* lds)
 {
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     const int i = c.perm[s];
     if (!(c.sflags[s] & kDynamic)) {
-        st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
-        st3(a.v_out, i, ld3(a.v_in, i));
-        st3(a.x_out, i, ld3(a.x_in, i));
+        if (lead) {
+            st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
+            st3(a.v_out, i, ld3(a.v_in, i));
+            st3(a.x_out, i, ld3(a.x_in, i));
+        }
         return;
     }
     const float4 xi4 = c.pos[s];
@@ -544,31 +610,10 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
     const float4 si = a.sca[s];
     const float dp_i = si.z, m_i = xi4.w;
     const float mu = a.viscosity[i], gamma = a.surface_tension[i];
-    const float h = c.h;
-    const Smooth sm = smooth_of(h);
-    const float visc_den0 = 0.01f * (h * h);
-    const float mu10 = 10.0f * mu;
-    v3 acc = V(0.0f, 0.0f, 0.0f);
-    REPLAY_NEIGHBORS_BU(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
-        [&](int k) { return NbrXVS{c.sflags[k], c.pos[k], a.vel[k], a.sca[k].z}; },
-        [&](int k, const NbrXVS& nb) {
-            if (k == s || !(nb.flags & kActive)) return;
-            const float4 xj = nb.x;
-            const v3 d = vsub(xi, xyz(xj));
-            const LenGrad lg = cubic_len_grad(d, sm);
-            const float distance = lg.len;
-            const float mass_nei = xj.w;
-            const v3 grad = lg.grad;
-            if (gamma != 0.0f)
-                acc = vsub(acc, muls(smul(gamma / m_i * mass_nei, d), cubic_w(fmax_(distance, pr), sm)));
-            if (mu != 0.0f) {
-                const float4 vj = nb.v;
-                float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
-                float sv = mu10 * vj.w * v_current_nei / (distance * distance + visc_den0);
-                acc = vadd(acc, smul(sv, grad));
-            }
-            acc = vadd(acc, smul(-mass_nei * (dp_i + nb.s), grad));
-        }));
+    const v3 acc = gamma != 0.0f
+                       ? wcsph_force_sum<BLOCK, CAP, UNR, LPP, true>(a, c, pr, s, xi, vi, dp_i, m_i, mu, gamma, lds)
+                       : wcsph_force_sum<BLOCK, CAP, UNR, LPP, false>(a, c, pr, s, xi, vi, dp_i, m_i, mu, gamma, lds);
+    if (!lead) return;
     int w = a.world[i];
     w = w > 0 ? w : 0;
     const v3 acc_g = vadd(acc, ld3(a.gravity, w));
@@ -586,23 +631,26 @@ __device__ __forceinline__ void wcsph_force_lane(const WcsphForceArgs& a, const 
 }
 
 // PPT particles per lane: the workgroup covers PPT consecutive tiles of BLOCK
-// slots and each lane takes one slot of each, one after another
-template <int BLOCK, int CAP, bool BATCH, int UNR, int PPT>
+// slots and each lane takes one slot of each, one after another.  LPP lanes
+// per particle (PPT = 1): the workgroup covers BLOCK / LPP slots.
+template <int BLOCK, int CAP, bool BATCH, int UNR, int PPT, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_wcsph_force(WcsphForceArgs a)
 {
+    static_assert(PPT == 1 || LPP == 1, "particles per lane or lanes per particle");
     __shared__ int lds[BLOCK * CAP];
     const int tile = xcd_tile();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     for (int q = 0; q < PPT; ++q) {
-        const int s = (tile * PPT + q) * BLOCK + threadIdx.x;
+        const int s = LPP > 1 ? pass_slot_g<BLOCK, LPP>() : (tile * PPT + q) * BLOCK + (int)threadIdx.x;
         if (s >= a.c.n) return;
         if constexpr (BATCH) {
             FOR_MY_SCENE(a, s, sc, {
                 if (sc.dt != 0.0f) {
                     PassCommon c = a.c;
                     scene_view(c, sc);
-                    wcsph_force_lane<BLOCK, CAP, UNR>(a, c, sc.particle_radius, sc.dt, sc.boundary_damping,
-                                                      sc.bound[0], sc.bound[1], sc.bound[2], s, lds);
-                } else {   // idle scene: carry the state over, keep its acceleration
+                    wcsph_force_lane<BLOCK, CAP, UNR, LPP>(a, c, sc.particle_radius, sc.dt, sc.boundary_damping,
+                                                           sc.bound[0], sc.bound[1], sc.bound[2], s, lds);
+                } else if (lead) {   // idle scene: carry the state over, keep its acceleration
                     const int i = a.c.perm[s];
                     st3(a.v_out, i, ld3(a.v_in, i));
                     st3(a.x_out, i, ld3(a.x_in, i));
@@ -611,13 +659,15 @@ __global__ __launch_bounds__(BLOCK) void k_wcsph_force(WcsphForceArgs a)
         } else {
             const float dt = a.dt_device ? *a.dt_device : a.dt;
             if (dt == 0.0f) {   // idle substep (device dt 0): carry the state over
-                const int i = a.c.perm[s];
-                st3(a.v_out, i, ld3(a.v_in, i));
-                st3(a.x_out, i, ld3(a.x_in, i));
+                if (lead) {
+                    const int i = a.c.perm[s];
+                    st3(a.v_out, i, ld3(a.v_in, i));
+                    st3(a.x_out, i, ld3(a.x_in, i));
+                }
                 continue;
             }
-            wcsph_force_lane<BLOCK, CAP, UNR>(a, a.c, a.particle_radius, dt, a.damping, a.bound[0], a.bound[1],
-                                              a.bound[2], s, lds);
+            wcsph_force_lane<BLOCK, CAP, UNR, LPP>(a, a.c, a.particle_radius, dt, a.damping, a.bound[0], a.bound[1],
+                                                   a.bound[2], s, lds);
         }
     }
 }
@@ -934,50 +984,82 @@ struct DfNonPressureArgs {
     float4* vstar_s;
 };
 
+// a neighbour's updates of the non-pressure acceleration, in this order:
+// cohesion (COH only: surface tension != 0), then viscosity
+template <bool COH>
+struct NonPressureTerms {
+    v3 visc;
+};
+template <>
+struct NonPressureTerms<true> {
+    v3 coh, visc;
+};
+
+template <int BLOCK, int CAP, int UNR, int LPP, bool COH>
+__device__ __forceinline__ v3 df_nonpressure_sum(const DfNonPressureArgs& a, int s, v3 xi, v3 vi, float m_i,
+                                                 float mu, float gamma, int* lds)
+{
+    const float h = a.c.h, pr = a.particle_radius;
+    const Smooth sm = smooth_of(h);
+    const float visc_den0 = 0.01f * (h * h);
+    const float mu10 = 10.0f * mu;
+    v3 acc = V(0.0f, 0.0f, 0.0f);
+    // zero terms (skipped neighbours, mu = 0) leave the sums unchanged
+    for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
+        a.c.g, a.c.pos, s, xi.x, xi.y, xi.z, a.c.h, a.c.cut2, lds, a.c.L,
+        [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vel[k]}; },
+        [&](int k, const NbrXV& nb) -> NonPressureTerms<COH> {
+            NonPressureTerms<COH> t;
+            if constexpr (COH) t.coh = V(0.0f, 0.0f, 0.0f);
+            t.visc = V(0.0f, 0.0f, 0.0f);
+            if (k == s || !(nb.flags & kActive)) return t;
+            const float4 xj = nb.x;
+            const v3 d = vsub(xi, xyz(xj));
+            const LenGrad lg = cubic_len_grad(d, sm);
+            const float dist = lg.len;
+            if constexpr (COH) t.coh = muls(smul(gamma / m_i * xj.w, d), cubic_w(fmax_(dist, pr), sm));
+            if (mu != 0.0f) {
+                const float4 vj = nb.v;
+                float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
+                float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
+                t.visc = smul(sv, lg.grad);
+            }
+            return t;
+        },
+        [&](const NonPressureTerms<COH>& t) {
+            if constexpr (COH) acc = vsub(acc, t.coh);
+            acc = vadd(acc, t.visc);
+        });
+    return acc;
+}
+
 // acceleration_non_pressure + predict_velocity  solver_dfsph.py:177-242
-template <int BLOCK, int CAP, int UNR>
+template <int BLOCK, int CAP, int UNR, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_dfsph_nonpressure(DfNonPressureArgs a)
 {
     __shared__ int lds[BLOCK * CAP];
-    const int s = pass_slot_b<BLOCK>();
+    const int s = pass_slot_g<BLOCK, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     if (s >= a.c.n) return;
     float dt;
     if (!substep_dt(a.dt_device, a.dt, dt)) return;
     const int i = a.c.perm[s];
     if (!(a.c.sflags[s] & kDynamic)) {
-        st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
-        st3(a.v_star, i, V(0.0f, 0.0f, 0.0f));
-        a.vstar_s[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (lead) {
+            st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
+            st3(a.v_star, i, V(0.0f, 0.0f, 0.0f));
+            a.vstar_s[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         return;
     }
     const float4 xi4 = a.c.pos[s];
     const v3 xi = xyz(xi4);
     const v3 vi = xyz(a.vel[s]);
     const float m_i = xi4.w, mu = a.viscosity[i], gamma = a.surface_tension[i];
-    const float h = a.c.h, pr = a.particle_radius;
-    const Smooth sm = smooth_of(h);
-    const float visc_den0 = 0.01f * (h * h);
-    const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
-    if (gamma != 0.0f || mu != 0.0f) {
-        REPLAY_NEIGHBORS_BU(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
-            [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vel[k]}; },
-            [&](int k, const NbrXV& nb) {
-                if (k == s || !(nb.flags & kActive)) return;
-                const float4 xj = nb.x;
-                const v3 d = vsub(xi, xyz(xj));
-                const LenGrad lg = cubic_len_grad(d, sm);
-                const float dist = lg.len;
-                if (gamma != 0.0f)
-                    acc = vsub(acc, muls(smul(gamma / m_i * xj.w, d), cubic_w(fmax_(dist, pr), sm)));
-                if (mu != 0.0f) {
-                    const float4 vj = nb.v;
-                    float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
-                    float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
-                    acc = vadd(acc, smul(sv, lg.grad));
-                }
-            }));
-    }
+    if (gamma != 0.0f) acc = df_nonpressure_sum<BLOCK, CAP, UNR, LPP, true>(a, s, xi, vi, m_i, mu, gamma, lds);
+    else if (mu != 0.0f) acc = df_nonpressure_sum<BLOCK, CAP, UNR, LPP, false>(a, s, xi, vi, m_i, mu, gamma, lds);
+    if (!lead) return;
     int w = a.world[i];
     w = w > 0 ? w : 0;
     const v3 acc_g = vadd(acc, ld3(a.gravity, w));
@@ -1323,54 +1405,78 @@ struct PbfNonPressureArgs {
 
 // acceleration_non_pressure (PBF) + apply_non_pressure_prediction +
 // apply_position_bounds  solver_pbf.py:144-189, 227-242, 784-796
-template <int BLOCK, int CAP, int UNR>
+template <int BLOCK, int CAP, int UNR, int LPP, bool COH>
+__device__ __forceinline__ v3 pbf_nonpressure_sum(const PbfNonPressureArgs& a, int s, v3 xi, v3 vi, float m_i,
+                                                  float mu, float gamma, int* lds)
+{
+    const float h = a.c.h, pr = a.particle_radius;
+    const float visc_den0 = 0.01f * (h * h);
+    const float mu10 = 10.0f * mu;
+    v3 acc = V(0.0f, 0.0f, 0.0f);
+    // zero terms (skipped neighbours, mu = 0, no relative motion) leave the
+    // sums unchanged
+    for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
+        a.c.g, a.c.pos, s, xi.x, xi.y, xi.z, a.c.h, a.c.cut2, lds, a.c.L,
+        [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vel[k]}; },
+        [&](int k, const NbrXV& nb) -> NonPressureTerms<COH> {
+            NonPressureTerms<COH> t;
+            if constexpr (COH) t.coh = V(0.0f, 0.0f, 0.0f);
+            t.visc = V(0.0f, 0.0f, 0.0f);
+            if (k == s || !(nb.flags & kActive)) return t;
+            const float4 xj = nb.x;
+            const v3 d = vsub(xi, xyz(xj));
+            const float dist = vlen(d);
+            if constexpr (COH) t.coh = muls(smul(gamma / m_i * xj.w, d), poly6_w(fmax_(dist, pr), h));
+            if (mu != 0.0f) {
+                const float4 vj = nb.v;
+                float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
+                if (v_current_nei != 0.0f) {
+                    float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
+                    t.visc = smul(sv, spiky_grad_r(d, dist, h));
+                }
+            }
+            return t;
+        },
+        [&](const NonPressureTerms<COH>& t) {
+            if constexpr (COH) acc = vsub(acc, t.coh);
+            acc = vadd(acc, t.visc);
+        });
+    return acc;
+}
+
+template <int BLOCK, int CAP, int UNR, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_pbf_nonpressure(PbfNonPressureArgs a)
 {
     __shared__ int lds[BLOCK * CAP];
-    const int s = pass_slot_b<BLOCK>();
+    const int s = pass_slot_g<BLOCK, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     if (s < a.c.n) {
         const int i = a.c.perm[s];
         const float4 xi4 = a.c.pos[s];
         if (!(a.c.sflags[s] & kDynamic)) {
-            st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
-            a.pos_next[s] = xi4;
+            if (lead) {
+                st3(a.accel, i, V(0.0f, 0.0f, 0.0f));
+                a.pos_next[s] = xi4;
+            }
         } else {
             const v3 xi = xyz(xi4);
             const float4 vi4 = a.vel[s];
             const v3 vi = xyz(vi4);
             const float m_i = xi4.w, mu = a.viscosity[i], gamma = a.surface_tension[i];
-            const float h = a.c.h, pr = a.particle_radius;
-            const float visc_den0 = 0.01f * (h * h);
-            const float mu10 = 10.0f * mu;
             v3 acc = V(0.0f, 0.0f, 0.0f);
-            if (gamma != 0.0f || mu != 0.0f) {
-                REPLAY_NEIGHBORS_BU(a.c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
-                    [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vel[k]}; },
-                    [&](int k, const NbrXV& nb) {
-                        if (k == s || !(nb.flags & kActive)) return;
-                        const float4 xj = nb.x;
-                        const v3 d = vsub(xi, xyz(xj));
-                        const float dist = vlen(d);
-                        if (gamma != 0.0f)
-                            acc = vsub(acc, muls(smul(gamma / m_i * xj.w, d), poly6_w(fmax_(dist, pr), h)));
-                        if (mu != 0.0f) {
-                            const float4 vj = nb.v;
-                            float v_current_nei = vdot(vsub(vi, xyz(vj)), d);
-                            if (v_current_nei != 0.0f) {
-                                float sv = mu10 * vj.w * v_current_nei / (dist * dist + visc_den0);
-                                acc = vadd(acc, smul(sv, spiky_grad_r(d, dist, h)));
-                            }
-                        }
-                    }));
+            if (gamma != 0.0f) acc = pbf_nonpressure_sum<BLOCK, CAP, UNR, LPP, true>(a, s, xi, vi, m_i, mu, gamma, lds);
+            else if (mu != 0.0f)
+                acc = pbf_nonpressure_sum<BLOCK, CAP, UNR, LPP, false>(a, s, xi, vi, m_i, mu, gamma, lds);
+            if (lead) {
+                st3(a.accel, i, acc);
+                const v3 delta_v = smul(a.dt, acc);
+                const v3 v = vadd(vi, delta_v);
+                v3 x = vadd(xi, smul(a.dt, delta_v));
+                x = a.bounds.clamp(x);
+                st3(a.v_pred, i, v);
+                st3(a.x_pred, i, x);
+                a.pos_next[s] = make_float4(x.x, x.y, x.z, xi4.w);
             }
-            st3(a.accel, i, acc);
-            const v3 delta_v = smul(a.dt, acc);
-            const v3 v = vadd(vi, delta_v);
-            v3 x = vadd(xi, smul(a.dt, delta_v));
-            x = a.bounds.clamp(x);
-            st3(a.v_pred, i, v);
-            st3(a.x_pred, i, x);
-            a.pos_next[s] = make_float4(x.x, x.y, x.z, xi4.w);
         }
     }
 }
@@ -1448,37 +1554,44 @@ struct PbfDeltaArgs {
 
 // compute_delta_position + apply_delta_position (+ apply_position_bounds after
 // the last iteration)  solver_pbf.py:305-363, 860-872
-template <int BLOCK, int CAP, int UNR, bool SCORR>
+template <int BLOCK, int CAP, int UNR, bool SCORR, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_pbf_delta(PbfDeltaArgs a)
 {
     __shared__ int lds[BLOCK * CAP];
     PassCommon c = a.c;
-    const int s = pass_slot_b<BLOCK>();
+    const int s = pass_slot_g<BLOCK, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     if (s < c.n) {
         const int i = c.perm[s];
         const float4 xi4 = c.pos[s];
         if (!(c.sflags[s] & kDynamic)) {
-            st3(a.delta_p, i, V(0.0f, 0.0f, 0.0f));
-            a.pos_next[s] = xi4;
-            st3(a.x_pred, i, xyz(xi4));
+            if (lead) {
+                st3(a.delta_p, i, V(0.0f, 0.0f, 0.0f));
+                a.pos_next[s] = xi4;
+                st3(a.x_pred, i, xyz(xi4));
+            }
         } else {
             const v3 xi = xyz(xi4);
             const float rho0_i = fmax_(a.rest_density[i], 1.0e-6f);
             const float lam_i = a.sca[s].x;
             const float h = c.h;
             v3 dp = V(0.0f, 0.0f, 0.0f);
-            REPLAY_NEIGHBORS_BU(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
+            // skipped neighbours contribute +0, which leaves the sums unchanged
+            for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
+                c.g, c.pos, s, xi.x, xi.y, xi.z, c.h, c.cut2, lds, c.L,
                 [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.sca[k].x}; },
-                [&](int k, const NbrXS& nb) {
-                    if (k == s || !(nb.flags & kActive)) return;
+                [&](int k, const NbrXS& nb) -> v3 {
+                    if (k == s || !(nb.flags & kActive)) return V(0.0f, 0.0f, 0.0f);
                     const float4 xj = nb.x;
                     v3 xij = vsub(xi, xyz(xj));
                     const float rn = vlen(xij);
                     v3 grad = spiky_grad_r(xij, rn, h);
                     float scorr = 0.0f;
                     if (SCORR) scorr = compute_scorr(rn, h, a.scorr_k, a.scorr_n, a.scorr_w_delta_q);
-                    dp = vadd(dp, smul((lam_i + nb.s + scorr) * xj.w, grad));
-                }));
+                    return smul((lam_i + nb.s + scorr) * xj.w, grad);
+                },
+                [&](const v3& t) { dp = vadd(dp, t); });
+            if (!lead) return;
             dp = vdivs(dp, rho0_i);
             if (a.max_delta_position > 0.0f) {
                 float dp_len = vlen(dp);
@@ -1504,32 +1617,39 @@ struct PbfXsphArgs {
 };
 
 // apply_xsph_viscosity + copies + apply_bounds  solver_pbf.py:434-463, 919-954
-template <int BLOCK, int CAP, int UNR>
+template <int BLOCK, int CAP, int UNR, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_pbf_xsph(PbfXsphArgs a)
 {
     __shared__ int lds[BLOCK * CAP];
     PassCommon c = a.c;
-    const int s = pass_slot_b<BLOCK>();
+    const int s = pass_slot_g<BLOCK, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     if (s >= c.n) return;
     const int i = c.perm[s];
     v3 x = xyz(c.pos[s]);
     const v3 vi = xyz(a.vel[s]);
     if (!(c.sflags[s] & kDynamic)) {
-        st3(a.x_out, i, x);
-        st3(a.v_out, i, vi);
+        if (lead) {
+            st3(a.x_out, i, x);
+            st3(a.v_out, i, vi);
+        }
         return;
     }
     const float h = c.h;
     v3 corr = V(0.0f, 0.0f, 0.0f);
-    REPLAY_NEIGHBORS_BU(c, s, x.x, x.y, x.z, NoCount{}, phased(
+    // skipped neighbours contribute +0, which leaves the sums unchanged
+    for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
+        c.g, c.pos, s, x.x, x.y, x.z, c.h, c.cut2, lds, c.L,
         [&](int k) { return NbrXV{c.sflags[k], c.pos[k], a.vel[k]}; },
-        [&](int k, const NbrXV& nb) {
-            if (k == s || !(nb.flags & kActive)) return;
+        [&](int k, const NbrXV& nb) -> v3 {
+            if (k == s || !(nb.flags & kActive)) return V(0.0f, 0.0f, 0.0f);
             const float4 xj = nb.x;
             const float4 vj = nb.v;
             float w = poly6_w(vlen(vsub(x, xyz(xj))), h);
-            corr = vadd(corr, muls(smul(vj.w, vsub(xyz(vj), vi)), w));
-        }));
+            return muls(smul(vj.w, vsub(xyz(vj), vi)), w);
+        },
+        [&](const v3& t) { corr = vadd(corr, t); });
+    if (!lead) return;
     v3 v = vadd(vi, smul(a.xsph_c, corr));
     a.bounds.clamp_damp(x, v, a.damping);
     st3(a.x_out, i, x);
@@ -1618,16 +1738,18 @@ struct Variant {
 constexpr Variant kScanVariants[] = {{64, 48, 4, 1},  {64, 24, 4, 1},  {64, 32, 4, 1},  {64, 64, 4, 1},
                                      {128, 24, 4, 1}, {128, 32, 4, 1}, {128, 16, 4, 1}, {256, 32, 4, 1},
                                      {256, 24, 4, 1}, {256, 16, 4, 1}, {64, 40, 4, 1},  {64, 56, 4, 1}};
-constexpr Variant kReplayVariants[] = {{64, 16, 4, 1},  {128, 16, 4, 1}, {256, 16, 4, 1}, {512, 16, 4, 1},
-                                       {64, 16, 8, 1},  {256, 16, 8, 1}, {64, 16, 4, 2},  {256, 16, 4, 2}};
+constexpr Variant kReplayVariants[] = {{64, 16, 4, 1},     {128, 16, 4, 1},    {256, 16, 4, 1},    {512, 16, 4, 1},
+                                       {64, 16, 8, 1},     {256, 16, 8, 1},    {64, 16, 4, 2},     {256, 16, 4, 2},
+                                       {256, 16, 4, 1, 4}, {256, 16, 2, 1, 4}, {128, 16, 4, 1, 2}, {256, 16, 4, 1, 2}};
 // the scanning (density/factor, density, lambda) and list-replaying passes of
 // DFSPH and PBF
 constexpr Variant kSolverScanVariants[] = {{64, 48, 4, 1}, {64, 40, 4, 1}, {64, 56, 4, 1}, {64, 32, 4, 1},
                                            {128, 32, 4, 1}};
-constexpr Variant kSolverReplayVariants[] = {{256, 16, 4, 1}, {64, 16, 4, 1}, {128, 16, 4, 1},
-                                             {512, 16, 4, 1}, {256, 16, 8, 1}, {64, 16, 8, 1}};
-// DFSPH's list-replaying passes: the rate and correction passes also run with
-// 2 or 4 lanes per particle (the non-pressure pass with one)
+constexpr Variant kSolverReplayVariants[] = {{256, 16, 4, 1},    {64, 16, 4, 1},     {128, 16, 4, 1},
+                                             {512, 16, 4, 1},    {256, 16, 8, 1},    {64, 16, 8, 1},
+                                             {256, 16, 4, 1, 4}, {256, 16, 2, 1, 4}, {256, 16, 4, 1, 2},
+                                             {128, 16, 4, 1, 2}};
+// DFSPH's list-replaying passes (non-pressure, rate, correction)
 constexpr Variant kDfsphReplayVariants[] = {
     {256, 16, 4, 1, 1}, {64, 16, 4, 1, 1},  {128, 16, 4, 1, 1}, {512, 16, 4, 1, 1},
     {256, 16, 8, 1, 1}, {64, 16, 8, 1, 1},  {256, 16, 4, 1, 4}, {256, 16, 2, 1, 4},
@@ -1708,8 +1830,8 @@ template <bool BATCH>
 static int32_t launch_force(int v, int n, hipStream_t s, const WcsphForceArgs& a)
 {
     return with_variant<kTabReplay>(v, [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_wcsph_force<V.block, V.cap, BATCH, V.unroll, V.ppt>), dim3(blocks_for(n, V.block * V.ppt)),
-                  dim3(V.block), 0, s, a);
+        FS_LAUNCH((k_wcsph_force<V.block, V.cap, BATCH, V.unroll, V.ppt, V.lpp>),
+                  dim3(blocks_for(n, V.block * V.ppt / V.lpp)), dim3(V.block), 0, s, a);
         return FS_OK;
     });
 }
@@ -2298,8 +2420,8 @@ static int32_t df_nonpressure_predict(const DfPlan& p)
     npa.v_star = p.io->v_star;
     npa.vstar_s = p.vstar_s;
     return with_variant<kTabDfsphReplay>(tuned(kTuneDfsphReplay), [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_dfsph_nonpressure<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0,
-                  p.s, npa);
+        FS_LAUNCH((k_dfsph_nonpressure<V.block, V.cap, V.unroll, V.lpp>), dim3(blocks_for(p.n, V.block / V.lpp)),
+                  dim3(V.block), 0, p.s, npa);
         return FS_OK;
     });
 }
@@ -2684,8 +2806,8 @@ static int32_t pbf_nonpressure(const PbfPlan& p)
     na.v_pred = p.w->v_pred;
     na.pos_next = p.bufs[1];
     return with_variant<kTabSolverReplay>(tuned(kTunePbfReplay), [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_pbf_nonpressure<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0,
-                  p.s, na);
+        FS_LAUNCH((k_pbf_nonpressure<V.block, V.cap, V.unroll, V.lpp>), dim3(blocks_for(p.n, V.block / V.lpp)),
+                  dim3(V.block), 0, p.s, na);
         return FS_OK;
     });
 }
@@ -2730,11 +2852,11 @@ static int32_t pbf_delta(const PbfPlan& p, int it)
     pa.pos_next = p.bufs[cur ^ 1];
     int32_t rc = with_variant<kTabSolverReplay>(tuned(kTunePbfReplay), [&]<Variant V>() -> int32_t {
         if (prm->scorr_k != 0.0f)
-            FS_LAUNCH((k_pbf_delta<V.block, V.cap, V.unroll, true>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0,
-                      p.s, pa);
+            FS_LAUNCH((k_pbf_delta<V.block, V.cap, V.unroll, true, V.lpp>), dim3(blocks_for(p.n, V.block / V.lpp)),
+                      dim3(V.block), 0, p.s, pa);
         else
-            FS_LAUNCH((k_pbf_delta<V.block, V.cap, V.unroll, false>), dim3(blocks_for(p.n, V.block)), dim3(V.block),
-                      0, p.s, pa);
+            FS_LAUNCH((k_pbf_delta<V.block, V.cap, V.unroll, false, V.lpp>), dim3(blocks_for(p.n, V.block / V.lpp)),
+                      dim3(V.block), 0, p.s, pa);
         return FS_OK;
     });
     if (rc) return rc;
@@ -2761,7 +2883,8 @@ static int32_t pbf_xsph_finalize(const PbfPlan& p)
     xa.x_out = p.io->x_out;
     xa.v_out = p.io->v_out;
     return with_variant<kTabSolverReplay>(tuned(kTunePbfReplay), [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_pbf_xsph<V.block, V.cap, V.unroll>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s, xa);
+        FS_LAUNCH((k_pbf_xsph<V.block, V.cap, V.unroll, V.lpp>), dim3(blocks_for(p.n, V.block / V.lpp)), dim3(V.block),
+                  0, p.s, xa);
         return FS_OK;
     });
 }

@@ -309,38 +309,42 @@ def test_list_capacity_does_not_change_results(method):
             assert np.array_equal(o[k].view(np.uint32), ref), f"{method} {name}: capacity {cap} differs"
 
 
-def test_dfsph_lane_groups_with_overflowing_lists():
-    """DFSPH's rate and correction passes under 2 and 4 lanes per particle:
-    the group's lanes evaluate alternate list entries and fold them in list
+@pytest.mark.parametrize("method,pass_name", [("wcsph", "wcsph_force"), ("dfsph", "dfsph_replay"),
+                                              ("pbf", "pbf_replay")])
+def test_lane_groups_with_overflowing_lists(method, pass_name):
+    """The list-replaying passes under 2 and 4 lanes per particle: the
+    group's lanes evaluate alternate list entries and fold them in list
     order; a particle whose list overflowed is scanned by the group's first
     lane alone.  With capacity 0 (every pass scans), 8 (most lists overflow)
     and 96, every lanes-per-particle variant matches the one-lane default bit
     for bit."""
     from fluid_simulation_amd import autotune as at
 
-    model, cfg = _compressed_case("dfsph", seed=3)
-    shapes = at.describe("dfsph_replay")
+    model, cfg = _compressed_case(method, seed=3, surface_tension=0.5 if method == "wcsph" else 0.0)
+    dt = 2.0e-4 if method == "wcsph" else 1.0e-3
+    shapes = at.describe(pass_name)
     picks = [v for v, sh in enumerate(shapes) if sh["lanes_per_particle"] > 1]
     assert {shapes[v]["lanes_per_particle"] for v in picks} == {2, 4}
+    extra = {"wcsph": ("pressure",), "dfsph": ("kappa", "kappa_v"), "pbf": ("pbf_lambda",)}[method]
     try:
         for cap in (0, 8, 96):
             outs = {}
             for v in [0] + picks:
-                at.select("dfsph_replay", v)
-                solver = make_solver("dfsph", model, cfg, neighbor_list_capacity=cap)
+                at.select(pass_name, v)
+                solver = make_solver(method, model, cfg, neighbor_list_capacity=cap)
                 s_in, s_out = model.state(), model.state()
                 for _ in range(3):
-                    solver.step(s_in, s_out, None, None, 1.0e-3)
+                    solver.step(s_in, s_out, None, None, dt)
                     s_in, s_out = s_out, s_in
                 torch.cuda.synchronize()
-                outs[v] = [s_in.particle_q.numpy(), s_in.particle_qd.numpy(), s_in.sph.rho.numpy(),
-                           s_in.sph.kappa.numpy(), s_in.sph.kappa_v.numpy()]
+                outs[v] = [s_in.particle_q.numpy(), s_in.particle_qd.numpy(), s_in.sph.rho.numpy()] + \
+                          [getattr(s_in.sph, name).numpy() for name in extra]
             for v in picks:
                 for k, (a, b) in enumerate(zip(outs[v], outs[0])):
                     assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), \
-                        f"capacity {cap}, variant {shapes[v]}: output {k} differs"
+                        f"{method} capacity {cap}, variant {shapes[v]}: output {k} differs"
     finally:
-        at.select("dfsph_replay", 0)
+        at.select(pass_name, 0)
 
 
 @pytest.mark.parametrize("method", ["wcsph", "dfsph", "pbf"])
