@@ -877,6 +877,155 @@ __device__ __forceinline__ void for_each_neighbor_split(const GridView& g, const
     }
 }
 
+// Sum of an integer over the lane's group of LPP lanes (every lane of the
+// group active).
+template <int LPP>
+__device__ __forceinline__ int group_sum(int x)
+{
+    if constexpr (LPP == 1) {
+        return x;
+    } else if constexpr (LPP == 2) {
+        return group_bcast<2, 0>(x) + group_bcast<2, 1>(x);
+    } else {
+        return (group_bcast<4, 0>(x) + group_bcast<4, 1>(x)) + (group_bcast<4, 2>(x) + group_bcast<4, 3>(x));
+    }
+}
+
+// Exclusive prefix (over the group's lower lanes) and total of an integer
+// over the lane's group of LPP lanes (every lane of the group active).
+template <int LPP>
+__device__ __forceinline__ int group_prefix(int x, int& total)
+{
+    const int q = (int)(threadIdx.x & (LPP - 1));
+    if constexpr (LPP == 2) {
+        const int x0 = group_bcast<2, 0>(x), x1 = group_bcast<2, 1>(x);
+        total = x0 + x1;
+        return q == 1 ? x0 : 0;
+    } else {
+        const int x0 = group_bcast<4, 0>(x), x1 = group_bcast<4, 1>(x);
+        const int x2 = group_bcast<4, 2>(x), x3 = group_bcast<4, 3>(x);
+        total = (x0 + x1) + (x2 + x3);
+        return (q > 0 ? x0 : 0) + (q > 1 ? x1 : 0) + (q > 2 ? x2 : 0);
+    }
+}
+
+// Scanning loop with LPP lanes per particle (LPP = 2 or 4).  The group walks
+// the query's rows together (every lane keeps the same row state); each trip
+// takes the next LPP * BL candidates of the current row, lane q testing
+// candidates k0 + q BL .. k0 + q BL + BL - 1, so lane order is visit order.
+// Survivors are compacted, in that order, into the group's LDS list (entry e
+// at lds[e * BLOCK / LPP + group]) at offsets from a prefix over the group.  The list is then
+// flushed as replay_split does a recorded list: lane q evaluates term() for
+// entries q, q + LPP, ... (U at a time) and the group folds the terms in
+// entry order; active survivors are recorded in order when L is in build
+// mode.  count(k) runs on the lane that tests candidate k (callers sum it
+// over the group).  Returns the number of candidates (the same on every lane
+// of the group).
+template <int BLOCK, int CAP, int LPP, int BL, int U, class Count, class Load, class Term, class Fold>
+__device__ __forceinline__ int scan_split(const GridView& g, const float4* __restrict__ pos, int s, float px,
+                                          float py, float pz, float r, float cut2, int* __restrict__ lds,
+                                          const NbrList& L, Count&& count, Load& load, Term& term, Fold& fold)
+{
+    static_assert(LPP == 2 || LPP == 4, "lanes per particle");
+    constexpr int kGroups = BLOCK / LPP;
+    constexpr int kTrip = LPP * BL;   // candidates per trip of the group
+    static_assert(CAP >= 2 * kTrip, "list capacity");
+    constexpr bool kCounts = !__is_same(typename std::decay<Count>::type, NoCount);
+    const int q = (int)(threadIdx.x & (LPP - 1));
+    int* __restrict__ list = lds + threadIdx.x / LPP;
+    const bool record = L.mode == kListBuild;
+    const float keep2 = record ? fmax_(L.cut2, cut2) : cut2;
+    const ptrdiff_t n = L.n;
+    QueryRows it;
+    it.begin(g, px, py, pz, r);
+    int written = 0;
+    int candidates = 0;
+    if (it.valid() && it.skip_empty(g)) {
+        const __amdgpu_buffer_rsrc_t rs = float4_rsrc(pos, g.npts);
+        bool done = false;
+        while (true) {
+            int cnt = 0;
+            while (!done && cnt <= CAP - kTrip) {
+                const int k0 = it.k;
+                const int rem = it.kend - k0;
+                const int take = rem < kTrip ? rem : kTrip;
+                const bool row_end = take == rem;
+                const int kq = k0 + q * BL;
+                float4 c[BL];
+                const int off = (int)((unsigned)kq * 16u);
+#pragma unroll
+                for (int u = 0; u < BL; ++u) c[u] = ld_f4(rs, off, u * 16);
+                it.k = k0 + take;
+                if (row_end && !it.advance(g)) done = true;
+                bool hit[BL];
+                int mine = 0;
+#pragma unroll
+                for (int u = 0; u < BL; ++u) {
+                    const bool in = q * BL + u < take;
+                    if (kCounts && in) count(kq + u);
+                    float ddx = px - c[u].x, ddy = py - c[u].y, ddz = pz - c[u].z;
+                    hit[u] = in && !(prefilter_d2(ddx, ddy, ddz) > keep2);
+                    mine += hit[u] ? 1 : 0;
+                }
+                // survivors of the group's lower lanes come first
+                int total;
+                int at = cnt + group_prefix<LPP>(mine, total);
+#pragma unroll
+                for (int u = 0; u < BL; ++u) {
+                    if (hit[u]) list[at * kGroups] = kq + u;
+                    at += hit[u] ? 1 : 0;
+                }
+                cnt += total;
+                candidates += take;
+                if (row_end && !done && !it.skip_empty(g)) done = true;
+            }
+            // the group's list entries are written by its own lanes (one
+            // wave); keep the reads below after those writes
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int e0 = 0; e0 < cnt; e0 += LPP * U) {
+                int k[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int e = e0 + u * LPP + q;
+                    k[u] = e < cnt ? list[e * kGroups] : s;
+                }
+                decltype(load(0)) d[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) d[u] = load(k[u]);
+                if (record) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int act = (e0 + u * LPP + q < cnt && (d[u].flags & kActive)) ? 1 : 0;
+                        int nact;
+                        const int p = written + group_prefix<LPP>(act, nact);
+                        if (act && p < L.cap) L.idx[s + (ptrdiff_t)p * n] = k[u];
+                        written += nact;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const auto t = term(k[u], d[u]);
+                    if (e0 + u * LPP < cnt) fold(group_bcast<LPP, 0>(t));
+                    if (e0 + u * LPP + 1 < cnt) fold(group_bcast<LPP, 1>(t));
+                    if constexpr (LPP == 4) {
+                        if (e0 + u * LPP + 2 < cnt) fold(group_bcast<LPP, 2>(t));
+                        if (e0 + u * LPP + 3 < cnt) fold(group_bcast<LPP, 3>(t));
+                    }
+                }
+            }
+            // the next trip's writes may reuse entries read above
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (done) break;
+        }
+    }
+    if (record && q == 0) L.cnt[s] = written <= L.cap ? written : L.cap + 1;
+    return candidates;
+}
+
 // Block index -> tile index so that each of the 8 XCDs (which receive
 // workgroups round-robin) processes one contiguous run of tiles.
 __device__ __forceinline__ int xcd_tile()

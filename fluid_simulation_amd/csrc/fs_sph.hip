@@ -125,6 +125,28 @@ __device__ __forceinline__ void scene_view(PassCommon& c, const fs_scene& sc)
 // a replaying pass templated on its block size and replay unroll (UNR)
 #define REPLAY_NEIGHBORS_BU(...) NBR_LOOP(BLOCK, true, UNR, __VA_ARGS__)
 
+// Scanning neighbour loop of a pass whose body is fold(term(k, data)): the
+// one-lane compacting scan (LPP = 1) or the group scan of LPP lanes per
+// particle (scan_split, U survivors per lane fetched together).  A term of a
+// skipped neighbour is zero, which leaves every sum unchanged (a
+// round-to-nearest sum from +0 never becomes -0).  Returns the number of
+// candidates.
+template <int BLOCK, int CAP, int LPP, int U, class Count, class Load, class Term, class Fold>
+__device__ __forceinline__ int scan_pass(const PassCommon& c, int s, float px, float py, float pz, int* lds,
+                                         Count&& count, Load load, Term term, Fold fold)
+{
+    if constexpr (LPP == 1) {
+        return for_each_neighbor<BLOCK, CAP, false>(
+            c.g, c.pos, s, px, py, pz, c.h, c.cut2, lds, c.L, count,
+            phased(load, [&](int k, const decltype(load(0))& d) { fold(term(k, d)); }));
+    } else {
+        NbrList scan = c.L;
+        if (scan.mode == kListUse) scan.mode = kListNone;
+        return scan_split<BLOCK, CAP, LPP, kBatch / LPP, U>(c.g, c.pos, s, px, py, pz, c.h, c.cut2, lds, scan, count,
+                                                              load, term, fold);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // DFSPH residual: each block of the rate pass stores its partial sums; a
 // one-block kernel then reduces them in a fixed order and decides the early exit
@@ -420,31 +442,37 @@ struct WcsphDensityArgs {
     float4* vel;    // .w <- m/max(rho,1e-6)
 };
 
-// compute_density_pressure  solver_wcsph.py:119-158, for lane s with the
-// pass parameters in `c`
-template <int BLOCK, int CAP>
+// compute_density_pressure  solver_wcsph.py:119-158, for slot s with the
+// pass parameters in `c` (LPP lanes per particle; the group's first lane
+// writes)
+template <int BLOCK, int CAP, int LPP, int U>
 __device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, const PassCommon& c, int clamp, int s,
                                                    int* lds)
 {
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     const int i = c.perm[s];
     const float4 xi = c.pos[s];
     if (!(c.sflags[s] & kActive)) {
-        a.rho[i] = 0.0f;
-        a.pressure[i] = 0.0f;
-        a.sca[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        a.vel[s].w = xi.w / fmax_(0.0f, 1.0e-6f);
-        if (c.L.mode == kListBuild) c.L.cnt[s] = 0;
+        if (lead) {
+            a.rho[i] = 0.0f;
+            a.pressure[i] = 0.0f;
+            a.sca[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            a.vel[s].w = xi.w / fmax_(0.0f, 1.0e-6f);
+            if (c.L.mode == kListBuild) c.L.cnt[s] = 0;
+        }
         return;
     }
     const Smooth sm = smooth_of(c.h);
     float rho_temp = 0.0f;
-    SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
-        [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.wvol[k]}; },
-        [&](int k, const NbrXS& nb) {
-            if (!(nb.flags & kActive)) return;
+    scan_pass<BLOCK, CAP, LPP, U>(
+        c, s, xi.x, xi.y, xi.z, lds, NoCount{}, [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.wvol[k]}; },
+        [&](int k, const NbrXS& nb) -> float {
+            if (!(nb.flags & kActive)) return 0.0f;
             v3 dist = vsub(xyz(xi), xyz(nb.x));
-            rho_temp += nb.s * cubic_w(vlen(dist), sm);
-        }));
+            return nb.s * cubic_w(vlen(dist), sm);
+        },
+        [&](float t) { rho_temp += t; });
+    if (!lead) return;
     const float rho_i = rho_temp;
     float p = a.stiffness[i] * (fpow_(rho_i / fmax_(a.rest_density[i], 1.0e-6f), a.exponent[i]) - 1.0f);
     if (clamp) p = fmax_(p, 0.0f);
@@ -474,27 +502,29 @@ __device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, co
         }                                                             \
     } while (0)
 
-template <int BLOCK, int CAP, bool BATCH>
+template <int BLOCK, int CAP, bool BATCH, int LPP, int U>
 __global__ __launch_bounds__(BLOCK) void k_wcsph_density(WcsphDensityArgs a)
 {
-    __shared__ int lds[BLOCK * CAP];
-    const int s = pass_slot_b<BLOCK>();
+    __shared__ int lds[BLOCK / LPP * CAP];
+    const int s = pass_slot_g<BLOCK, LPP>();
     if (s >= a.c.n) return;
     if constexpr (BATCH) {
         FOR_MY_SCENE(a, s, sc, {
             if (sc.dt != 0.0f) {
                 PassCommon c = a.c;
                 scene_view(c, sc);
-                wcsph_density_lane<BLOCK, CAP>(a, c, sc.clamp_negative_pressure, s, lds);
+                wcsph_density_lane<BLOCK, CAP, LPP, U>(a, c, sc.clamp_negative_pressure, s, lds);
             }
         });
     } else {
         if (a.dt_device && *a.dt_device == 0.0f) {   // idle substep: carry rho and p over
-            if (a.rho_in) a.rho[s] = a.rho_in[s];
-            if (a.pressure_in) a.pressure[s] = a.pressure_in[s];
+            if ((threadIdx.x & (LPP - 1)) == 0) {
+                if (a.rho_in) a.rho[s] = a.rho_in[s];
+                if (a.pressure_in) a.pressure[s] = a.pressure_in[s];
+            }
             return;
         }
-        wcsph_density_lane<BLOCK, CAP>(a, a.c, a.clamp, s, lds);
+        wcsph_density_lane<BLOCK, CAP, LPP, U>(a, a.c, a.clamp, s, lds);
     }
 }
 
@@ -896,25 +926,36 @@ struct DfDensityArgs {
 };
 
 // compute_density + compute_factor_dfsph  solver_dfsph.py:106-174 (one pass)
-template <int BLOCK, int CAP>
+// a neighbour's contributions to the density, the gradient sum and the sum
+// of squared gradients
+struct FactorTerms {
+    float rho;
+    v3 grad;
+    float grad_sq;
+};
+
+template <int BLOCK, int CAP, int LPP, int U>
 __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
 {
-    __shared__ int lds[BLOCK * CAP];
-    const int s = pass_slot_b<BLOCK>();
+    __shared__ int lds[BLOCK / LPP * CAP];
+    const int s = pass_slot_g<BLOCK, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     if (s >= a.c.n) return;
     float dt_unused;
     if (!substep_dt(a.dt_device, 1.0f, dt_unused)) return;
     const int i = a.c.perm[s];
     const float4 xi4 = a.c.pos[s];
     if (!(a.c.sflags[s] & kActive)) {
-        a.rho[i] = 0.0f;
-        a.factor[i] = 0.0f;
-        a.neighbor_count[i] = 0;
-        float kw = a.kappa_warm ? a.kappa_warm[i] : 0.0f;
-        if (a.kappa_warm) a.kappa_commit[i] = kw;
-        a.sca[s] = make_float4(0.0f, 0.0f, kw, kw / fmax_(0.0f, 1.0e-6f));
-        a.vel[s].w = xi4.w / fmax_(0.0f, 1.0e-6f);
-        if (a.c.L.mode == kListBuild) a.c.L.cnt[s] = 0;
+        if (lead) {
+            a.rho[i] = 0.0f;
+            a.factor[i] = 0.0f;
+            a.neighbor_count[i] = 0;
+            float kw = a.kappa_warm ? a.kappa_warm[i] : 0.0f;
+            if (a.kappa_warm) a.kappa_commit[i] = kw;
+            a.sca[s] = make_float4(0.0f, 0.0f, kw, kw / fmax_(0.0f, 1.0e-6f));
+            a.vel[s].w = xi4.w / fmax_(0.0f, 1.0e-6f);
+            if (a.c.L.mode == kListBuild) a.c.L.cnt[s] = 0;
+        }
         return;
     }
     const v3 xi = xyz(xi4);
@@ -923,17 +964,23 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
     float rho_temp = 0.0f;
     v3 sum_grad = V(0.0f, 0.0f, 0.0f);
     float sum_grad_sq = 0.0f;
-    auto body = phased([&](int k) { return NbrXS{a.c.sflags[k], a.c.pos[k], a.wvol[k]}; },
-                       [&](int k, const NbrXS& nb) {
-                           if (!(nb.flags & kActive)) return;
-                           const float4 xj = nb.x;
-                           v3 r = vsub(xi, xyz(xj));
-                           const LenGrad lg = cubic_len_grad(r, sm);
-                           rho_temp += nb.s * cubic_w(lg.len, sm);
-                           v3 grad = smul(xj.w, lg.grad);
-                           sum_grad = vadd(sum_grad, grad);
-                           sum_grad_sq += vdot(grad, grad);
-                       });
+    auto load = [&](int k) { return NbrXS{a.c.sflags[k], a.c.pos[k], a.wvol[k]}; };
+    auto term = [&](int k, const NbrXS& nb) -> FactorTerms {
+        FactorTerms t{0.0f, V(0.0f, 0.0f, 0.0f), 0.0f};
+        if (!(nb.flags & kActive)) return t;
+        const float4 xj = nb.x;
+        v3 r = vsub(xi, xyz(xj));
+        const LenGrad lg = cubic_len_grad(r, sm);
+        t.rho = nb.s * cubic_w(lg.len, sm);
+        t.grad = smul(xj.w, lg.grad);
+        t.grad_sq = vdot(t.grad, t.grad);
+        return t;
+    };
+    auto fold = [&](const FactorTerms& t) {
+        rho_temp += t.rho;
+        sum_grad = vadd(sum_grad, t.grad);
+        sum_grad_sq += t.grad_sq;
+    };
     // count = active candidates other than the particle itself, no distance
     // test (solver_dfsph.py:150-160)
     int count = 0;
@@ -947,14 +994,16 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_density_factor(DfDensityArgs a)
                          cell_coord((xi.z - h) * gv.inv, gv.round_mode) < -kOrigin;
     if (*a.any_inactive || repeats) {
         auto counter = [&](int k) { count += (k != s && (a.c.sflags[k] & kActive)) ? 1 : 0; };
-        SCAN_NEIGHBORS_B(a.c, s, xi.x, xi.y, xi.z, counter, body);
+        scan_pass<BLOCK, CAP, LPP, U>(a.c, s, xi.x, xi.y, xi.z, lds, counter, load, term, fold);
+        count = group_sum<LPP>(count);   // each lane counted the candidates it tested
     } else {
         // every particle active and no bucket repeated: all candidates but
         // the particle itself, which is among its own candidates exactly once
         // (its bucket lies in the query range)
-        const int total = SCAN_NEIGHBORS_B(a.c, s, xi.x, xi.y, xi.z, NoCount{}, body);
+        const int total = scan_pass<BLOCK, CAP, LPP, U>(a.c, s, xi.x, xi.y, xi.z, lds, NoCount{}, load, term, fold);
         count = total > 0 ? total - 1 : 0;
     }
+    if (!lead) return;
     const float denom = vdot(sum_grad, sum_grad) + sum_grad_sq;
     const float f = (count < a.min_neighbors || denom < 1.0e-6f) ? 0.0f : 1.0f / denom;
     a.rho[i] = rho_temp;
@@ -1345,12 +1394,13 @@ struct PbfDensityArgs {
 // compute_density (poly6)  solver_pbf.py:114-141; in the final pass also
 // update_velocity_from_positions :402-416 and, without XSPH, :939-954.
 // vel[s].w <- m/max(rho,1e-6) for the non-pressure and XSPH passes.
-template <int BLOCK, int CAP>
+template <int BLOCK, int CAP, int LPP, int U>
 __global__ __launch_bounds__(BLOCK) void k_pbf_density(PbfDensityArgs a)
 {
-    __shared__ int lds[BLOCK * CAP];
+    __shared__ int lds[BLOCK / LPP * CAP];
     PassCommon c = a.c;
-    const int s = pass_slot_b<BLOCK>();
+    const int s = pass_slot_g<BLOCK, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     if (s >= c.n) return;
     const int i = c.perm[s];
     const int fl = c.sflags[s];
@@ -1359,16 +1409,18 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_density(PbfDensityArgs a)
     float rho_temp = 0.0f;
     if (fl & kActive) {
         const float h = c.h;
-        SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
-            [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.wvol[k]}; },
-            [&](int k, const NbrXS& nb) {
-                if (!(nb.flags & kActive)) return;
+        scan_pass<BLOCK, CAP, LPP, U>(
+            c, s, xi.x, xi.y, xi.z, lds, NoCount{}, [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.wvol[k]}; },
+            [&](int k, const NbrXS& nb) -> float {
+                if (!(nb.flags & kActive)) return 0.0f;
                 v3 r = vsub(xi, xyz(nb.x));
-                rho_temp += nb.s * poly6_w(vlen(r), h);
-            }));
-    } else if (c.L.mode == kListBuild) {
+                return nb.s * poly6_w(vlen(r), h);
+            },
+            [&](float t) { rho_temp += t; });
+    } else if (c.L.mode == kListBuild && lead) {
         c.L.cnt[s] = 0;
     }
+    if (!lead) return;
     a.rho[i] = rho_temp;
     a.sca[s] = make_float4(rho_temp, 0.0f, 0.0f, 0.0f);
     const float mr = xi4.w / fmax_(rho_temp, 1.0e-6f);
@@ -1493,19 +1545,31 @@ struct PbfLambdaArgs {
 };
 
 // compute_pbf_lambda  solver_pbf.py:245-302
-template <int BLOCK, int CAP>
+// a neighbour's contributions to the PBF constraint: the neighbour count,
+// density, gradient and squared gradient
+struct LambdaTerms {
+    int count;
+    float rho;
+    v3 grad;
+    float grad_sq;
+};
+
+template <int BLOCK, int CAP, int LPP, int U>
 __global__ __launch_bounds__(BLOCK) void k_pbf_lambda(PbfLambdaArgs a)
 {
-    __shared__ int lds[BLOCK * CAP];
+    __shared__ int lds[BLOCK / LPP * CAP];
     PassCommon c = a.c;
-    const int s = pass_slot_b<BLOCK>();
+    const int s = pass_slot_g<BLOCK, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     if (s >= c.n) return;
     const int i = c.perm[s];
     if (!(c.sflags[s] & kDynamic)) {
-        a.pbf_lambda[i] = 0.0f;
-        a.rho[i] = 0.0f;
-        a.neighbor_count[i] = 0;
-        a.sca[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (lead) {
+            a.pbf_lambda[i] = 0.0f;
+            a.rho[i] = 0.0f;
+            a.neighbor_count[i] = 0;
+            a.sca[s] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         return;
     }
     const float rho0_i = fmax_(a.rest_density[i], 1.0e-6f);
@@ -1515,20 +1579,28 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_lambda(PbfLambdaArgs a)
     v3 grad_i = V(0.0f, 0.0f, 0.0f);
     float sum_grad_sq = 0.0f;
     int count = 0;
-    SCAN_NEIGHBORS_B(c, s, xi.x, xi.y, xi.z, NoCount{}, phased(
-        [&](int k) { return NbrX{c.sflags[k], c.pos[k]}; },
-        [&](int k, const NbrX& nb) {
-            if (!(nb.flags & kActive)) return;
+    scan_pass<BLOCK, CAP, LPP, U>(
+        c, s, xi.x, xi.y, xi.z, lds, NoCount{}, [&](int k) { return NbrX{c.sflags[k], c.pos[k]}; },
+        [&](int k, const NbrX& nb) -> LambdaTerms {
+            LambdaTerms t{0, 0.0f, V(0.0f, 0.0f, 0.0f), 0.0f};
+            if (!(nb.flags & kActive)) return t;
             const float4 xj = nb.x;
             v3 xij = vsub(xi, xyz(xj));
             float r = vlen(xij);
-            if (k != s && r < h) count += 1;
+            t.count = (k != s && r < h) ? 1 : 0;
             float mass_j = xj.w;
-            rho_i += mass_j * poly6_w(r, h);
-            v3 grad_j = smul(-(mass_j / rho0_i), spiky_grad_r(xij, r, h));
-            sum_grad_sq += vdot(grad_j, grad_j);
-            grad_i = vsub(grad_i, grad_j);
-        }));
+            t.rho = mass_j * poly6_w(r, h);
+            t.grad = smul(-(mass_j / rho0_i), spiky_grad_r(xij, r, h));
+            t.grad_sq = vdot(t.grad, t.grad);
+            return t;
+        },
+        [&](const LambdaTerms& t) {
+            count += t.count;
+            rho_i += t.rho;
+            sum_grad_sq += t.grad_sq;
+            grad_i = vsub(grad_i, t.grad);
+        });
+    if (!lead) return;
     sum_grad_sq += vdot(grad_i, grad_i);
     const float c_i = fmax_(rho_i / rho0_i - 1.0f, 0.0f);
     a.rho[i] = rho_i;
@@ -1735,16 +1807,19 @@ struct Variant {
     int lpp = 1;   // lanes per particle (block / lpp >= 64 particles per workgroup)
 };
 // variant 0 of each table is the default
-constexpr Variant kScanVariants[] = {{64, 48, 4, 1},  {64, 24, 4, 1},  {64, 32, 4, 1},  {64, 64, 4, 1},
-                                     {128, 24, 4, 1}, {128, 32, 4, 1}, {128, 16, 4, 1}, {256, 32, 4, 1},
-                                     {256, 24, 4, 1}, {256, 16, 4, 1}, {64, 40, 4, 1},  {64, 56, 4, 1}};
+constexpr Variant kScanVariants[] = {{64, 48, 4, 1},     {64, 24, 4, 1},     {64, 32, 4, 1},     {64, 64, 4, 1},
+                                     {128, 24, 4, 1},    {128, 32, 4, 1},    {128, 16, 4, 1},    {256, 32, 4, 1},
+                                     {256, 24, 4, 1},    {256, 16, 4, 1},    {64, 40, 4, 1},     {64, 56, 4, 1},
+                                     {128, 48, 4, 1, 2}, {128, 32, 4, 1, 2}, {128, 64, 2, 1, 2}, {256, 48, 2, 1, 4},
+                                     {256, 32, 2, 1, 4}};
 constexpr Variant kReplayVariants[] = {{64, 16, 4, 1},     {128, 16, 4, 1},    {256, 16, 4, 1},    {512, 16, 4, 1},
                                        {64, 16, 8, 1},     {256, 16, 8, 1},    {64, 16, 4, 2},     {256, 16, 4, 2},
                                        {256, 16, 4, 1, 4}, {256, 16, 2, 1, 4}, {128, 16, 4, 1, 2}, {256, 16, 4, 1, 2}};
 // the scanning (density/factor, density, lambda) and list-replaying passes of
 // DFSPH and PBF
-constexpr Variant kSolverScanVariants[] = {{64, 48, 4, 1}, {64, 40, 4, 1}, {64, 56, 4, 1}, {64, 32, 4, 1},
-                                           {128, 32, 4, 1}};
+constexpr Variant kSolverScanVariants[] = {{64, 48, 4, 1},     {64, 40, 4, 1},     {64, 56, 4, 1},
+                                           {64, 32, 4, 1},     {128, 32, 4, 1},    {128, 48, 4, 1, 2},
+                                           {128, 32, 4, 1, 2}, {256, 48, 2, 1, 4}, {256, 32, 2, 1, 4}};
 constexpr Variant kSolverReplayVariants[] = {{256, 16, 4, 1},    {64, 16, 4, 1},     {128, 16, 4, 1},
                                              {512, 16, 4, 1},    {256, 16, 8, 1},    {64, 16, 8, 1},
                                              {256, 16, 4, 1, 4}, {256, 16, 2, 1, 4}, {256, 16, 4, 1, 2},
@@ -1821,7 +1896,8 @@ template <bool BATCH>
 static int32_t launch_density(int v, int n, hipStream_t s, const WcsphDensityArgs& a)
 {
     return with_variant<kTabScan>(v, [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_wcsph_density<V.block, V.cap, BATCH>), dim3(blocks_for(n, V.block)), dim3(V.block), 0, s, a);
+        FS_LAUNCH((k_wcsph_density<V.block, V.cap, BATCH, V.lpp, V.unroll>), dim3(blocks_for(n, V.block / V.lpp)),
+                  dim3(V.block), 0, s, a);
         return FS_OK;
     });
 }
@@ -2397,8 +2473,8 @@ static int32_t df_density_factor(const DfPlan& p, bool after_integrate)
     dfa.vel = p.vel;
     dfa.dt_device = p.prm->dt_device;
     return with_variant<kTabSolverScan>(tuned(kTuneDfsphScan), [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_dfsph_density_factor<V.block, V.cap>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s,
-                  dfa);
+        FS_LAUNCH((k_dfsph_density_factor<V.block, V.cap, V.lpp, V.unroll>), dim3(blocks_for(p.n, V.block / V.lpp)),
+                  dim3(V.block), 0, p.s, dfa);
         return FS_OK;
     });
 }
@@ -2784,7 +2860,8 @@ static int32_t pbf_density(const PbfPlan& p, bool final_pass)
     da.x_out = p.io->x_out;
     da.v_out = p.io->v_out;
     return with_variant<kTabSolverScan>(tuned(kTunePbfScan), [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_pbf_density<V.block, V.cap>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s, da);
+        FS_LAUNCH((k_pbf_density<V.block, V.cap, V.lpp, V.unroll>), dim3(blocks_for(p.n, V.block / V.lpp)),
+                  dim3(V.block), 0, p.s, da);
         return FS_OK;
     });
 }
@@ -2825,7 +2902,8 @@ static int32_t pbf_lambda(const PbfPlan& p, int it)
     la.neighbor_count = p.io->neighbor_count;
     la.sca = p.sca;
     return with_variant<kTabSolverScan>(tuned(kTunePbfScan), [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_pbf_lambda<V.block, V.cap>), dim3(blocks_for(p.n, V.block)), dim3(V.block), 0, p.s, la);
+        FS_LAUNCH((k_pbf_lambda<V.block, V.cap, V.lpp, V.unroll>), dim3(blocks_for(p.n, V.block / V.lpp)),
+                  dim3(V.block), 0, p.s, la);
         return FS_OK;
     });
 }

@@ -310,14 +310,17 @@ def test_list_capacity_does_not_change_results(method):
 
 
 @pytest.mark.parametrize("method,pass_name", [("wcsph", "wcsph_force"), ("dfsph", "dfsph_replay"),
-                                              ("pbf", "pbf_replay")])
+                                              ("pbf", "pbf_replay"), ("wcsph", "wcsph_density"),
+                                              ("dfsph", "dfsph_scan"), ("pbf", "pbf_scan")])
 def test_lane_groups_with_overflowing_lists(method, pass_name):
-    """The list-replaying passes under 2 and 4 lanes per particle: the
+    """Passes under 2 and 4 lanes per particle.  Replaying passes: the
     group's lanes evaluate alternate list entries and fold them in list
     order; a particle whose list overflowed is scanned by the group's first
-    lane alone.  With capacity 0 (every pass scans), 8 (most lists overflow)
-    and 96, every lanes-per-particle variant matches the one-lane default bit
-    for bit."""
+    lane alone.  Scanning passes: the group tests consecutive candidates,
+    compacts the survivors in visit order, records them and folds their
+    terms in that order.  With capacity 0 (every pass scans), 8 (most lists
+    overflow) and 96, every lanes-per-particle variant matches the one-lane
+    default bit for bit."""
     from fluid_simulation_amd import autotune as at
 
     model, cfg = _compressed_case(method, seed=3, surface_tension=0.5 if method == "wcsph" else 0.0)
