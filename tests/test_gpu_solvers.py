@@ -319,20 +319,21 @@ def test_lane_groups_with_overflowing_lists(method, pass_name):
     lane alone.  Scanning passes: the group tests consecutive candidates,
     compacts the survivors in visit order, records them and folds their
     terms in that order.  With capacity 0 (every pass scans), 8 (most lists
-    overflow) and 96, every lanes-per-particle variant matches the one-lane
-    default bit for bit."""
+    overflow) and 96, every lanes-per-particle variant matches a one-lane
+    variant bit for bit."""
     from fluid_simulation_amd import autotune as at
 
     model, cfg = _compressed_case(method, seed=3, surface_tension=0.5 if method == "wcsph" else 0.0)
     dt = 2.0e-4 if method == "wcsph" else 1.0e-3
     shapes = at.describe(pass_name)
     picks = [v for v, sh in enumerate(shapes) if sh["lanes_per_particle"] > 1]
+    ref = next(v for v, sh in enumerate(shapes) if sh["lanes_per_particle"] == 1)
     assert {shapes[v]["lanes_per_particle"] for v in picks} == {2, 4}
     extra = {"wcsph": ("pressure",), "dfsph": ("kappa", "kappa_v"), "pbf": ("pbf_lambda",)}[method]
     try:
         for cap in (0, 8, 96):
             outs = {}
-            for v in [0] + picks:
+            for v in [ref] + picks:
                 at.select(pass_name, v)
                 solver = make_solver(method, model, cfg, neighbor_list_capacity=cap)
                 s_in, s_out = model.state(), model.state()
@@ -343,7 +344,7 @@ def test_lane_groups_with_overflowing_lists(method, pass_name):
                 outs[v] = [s_in.particle_q.numpy(), s_in.particle_qd.numpy(), s_in.sph.rho.numpy()] + \
                           [getattr(s_in.sph, name).numpy() for name in extra]
             for v in picks:
-                for k, (a, b) in enumerate(zip(outs[v], outs[0])):
+                for k, (a, b) in enumerate(zip(outs[v], outs[ref])):
                     assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), \
                         f"{method} capacity {cap}, variant {shapes[v]}: output {k} differs"
     finally:
