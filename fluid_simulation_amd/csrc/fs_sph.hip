@@ -781,18 +781,30 @@ struct WadjForceArgs {
     float* grad_viscosity;
 };
 
+// a neighbour's updates of the reverse force sums, applied in this order:
+// xb += x1, xb += x2, Pb += P, xb += x3, xb -= x4, vb += v, mub += mu, wb += w
+struct AdjForceTerms {
+    v3 x1, x2, x3, x4, v;
+    float P, mu, w;
+};
+
+// lanes per particle of the adjoint passes (few particles: more waves)
+constexpr int kAdjLpp = 4;
+
 // reverse of acceleration_wcsph (pressure + viscosity) and of the EOS
 // (solver_wcsph.py:92-106,119-228): per particle k, the pairs (k,j) of its
-// own acceleration and (j,k) where it is the neighbour.
-template <int CAP>
+// own acceleration and (j,k) where it is the neighbour.  LPP lanes per
+// particle share the neighbour terms; the group's first lane writes.
+template <int CAP, int LPP>
 __global__ __launch_bounds__(kPassBlock) void k_wadj_force(WadjForceArgs a)
 {
     __shared__ int lds[kPassBlock * CAP];
-    const int s = pass_slot();
+    const int s = pass_slot_g<kPassBlock, LPP>();
+    const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     if (s >= a.c.n) return;
     const PassCommon& c = a.c;
     if (!(c.sflags[s] & kActive)) {
-        a.rhobar[s] = 0.0f;
+        if (lead) a.rhobar[s] = 0.0f;
         return;
     }
     const int i = c.perm[s];
@@ -806,14 +818,18 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_force(WadjForceArgs a)
     const float eps2 = 0.01f * (h * h);
     v3 xb = V(0.0f, 0.0f, 0.0f), vb = V(0.0f, 0.0f, 0.0f);
     float Pb = 0.0f, wb = 0.0f, mub = 0.0f;
-    REPLAY_NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, phased(
+    // skipped neighbours give zero terms, which leave every sum unchanged
+    for_each_neighbor_split<kPassBlock, CAP, 2, LPP>(
+        c.g, c.pos, s, xk.x, xk.y, xk.z, c.h, c.cut2, lds, c.L,
         [&](int j) { return AdjNbr{c.sflags[j], c.pos[j], a.abar[j], a.vel[j], a.sca[j].z}; },
-        [&](int j, const AdjNbr& nb) {
-            if (j == s || !(nb.flags & kActive)) return;
+        [&](int j, const AdjNbr& nb) -> AdjForceTerms {
+            const v3 z = V(0.0f, 0.0f, 0.0f);
+            AdjForceTerms t{z, z, z, z, z, 0.0f, 0.0f, 0.0f};
+            if (j == s || !(nb.flags & kActive)) return t;
             const float4 xj4 = nb.x;
             const v3 r = vsub(xk, xyz(xj4));
             const float d = vlen(r);
-            if (!(d / h <= 1.0f)) return;
+            if (!(d / h <= 1.0f)) return t;
             const v3 G = cubic_grad_r(r, d, h);
             const float4 aj4 = nb.abar;
             const float4 vj4 = nb.v;
@@ -830,17 +846,29 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_force(WadjForceArgs a)
             const v3 Hak = cubic_hess_mul(r, d, h, ak);
             const v3 Haj = cubic_hess_mul(r, d, h, aj);
             const v3 ds = vdivs(vsub(dv, smul(2.0f * sv, r)), den);
-            xb = vadd(xb, smul(-m_j * (P_k + P_j), Hak));
-            xb = vadd(xb, smul(m_k * (P_j + P_k), Haj));
-            Pb += -m_j * Gak + m_k * Gaj;
+            t.x1 = smul(-m_j * (P_k + P_j), Hak);
+            t.x2 = smul(m_k * (P_j + P_k), Haj);
+            t.P = -m_j * Gak + m_k * Gaj;
             const float ck_wj = c_k * w_j;
             const float cj_wk = c_j * w_k;
-            xb = vadd(xb, smul(ck_wj, vadd(smul(Gak, ds), smul(sv, Hak))));
-            xb = vsub(xb, smul(cj_wk, vadd(smul(Gaj, ds), smul(sv, Haj))));
-            vb = vadd(vb, smul((ck_wj * Gak - cj_wk * Gaj) / den, r));
-            mub += 10.0f * w_j * sv * Gak;
-            wb += -c_j * sv * Gaj;
-        }));
+            t.x3 = smul(ck_wj, vadd(smul(Gak, ds), smul(sv, Hak)));
+            t.x4 = smul(cj_wk, vadd(smul(Gaj, ds), smul(sv, Haj)));
+            t.v = smul((ck_wj * Gak - cj_wk * Gaj) / den, r);
+            t.mu = 10.0f * w_j * sv * Gak;
+            t.w = -c_j * sv * Gaj;
+            return t;
+        },
+        [&](const AdjForceTerms& t) {
+            xb = vadd(xb, t.x1);
+            xb = vadd(xb, t.x2);
+            Pb += t.P;
+            xb = vadd(xb, t.x3);
+            xb = vsub(xb, t.x4);
+            vb = vadd(vb, t.v);
+            mub += t.mu;
+            wb += t.w;
+        });
+    if (!lead) return;
     float rb = 0.0f;
     const float pb = Pb / fmax_(rho_k * rho_k, 1.0e-6f);
     if (rho_k * rho_k > 1.0e-6f) rb += Pb * (-2.0f * p_k / (rho_k * rho_k * rho_k));
@@ -875,11 +903,11 @@ struct WadjDensityArgs {
 
 // reverse of the density sum of compute_density_pressure (solver_wcsph.py:119-158)
 // and the final write of dL/dx_in, dL/dv_in
-template <int CAP>
+template <int CAP, int LPP>
 __global__ __launch_bounds__(kPassBlock) void k_wadj_density(WadjDensityArgs a)
 {
     __shared__ int lds[kPassBlock * CAP];
-    const int s = pass_slot();
+    const int s = pass_slot_g<kPassBlock, LPP>();
     if (s >= a.c.n) return;
     const PassCommon& c = a.c;
     const int i = c.perm[s];
@@ -891,16 +919,19 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_density(WadjDensityArgs a)
         const float wv_k = a.wvol[s];
         const float rb_k = a.rhobar[s];
         const float h = c.h;
-        REPLAY_NEIGHBORS(c, s, xk.x, xk.y, xk.z, NoCount{}, phased(
+        for_each_neighbor_split<kPassBlock, CAP, kReplayUnroll, LPP>(
+            c.g, c.pos, s, xk.x, xk.y, xk.z, c.h, c.cut2, lds, c.L,
             [&](int j) { return NbrXV{c.sflags[j], c.pos[j], make_float4(a.wvol[j], a.rhobar[j], 0.0f, 0.0f)}; },
-            [&](int j, const NbrXV& nb) {
-                if (j == s || !(nb.flags & kActive)) return;
+            [&](int j, const NbrXV& nb) -> v3 {
+                if (j == s || !(nb.flags & kActive)) return V(0.0f, 0.0f, 0.0f);
                 const v3 r = vsub(xk, xyz(nb.x));
                 const float d = vlen(r);
-                if (!(d / h <= 1.0f)) return;
-                xb = vadd(xb, smul(rb_k * nb.v.x + nb.v.y * wv_k, cubic_grad_r(r, d, h)));
-            }));
+                if (!(d / h <= 1.0f)) return V(0.0f, 0.0f, 0.0f);
+                return smul(rb_k * nb.v.x + nb.v.y * wv_k, cubic_grad_r(r, d, h));
+            },
+            [&](const v3& t) { xb = vadd(xb, t); });
     }
+    if ((threadIdx.x & (LPP - 1)) != 0) return;
     st3(a.xbar_in, i, V(x0.x + xb.x, x0.y + xb.y, x0.z + xb.z));
     const float4 v0 = a.vbar_s[s];
     st3(a.vbar_in, i, V(v0.x, v0.y, v0.z));
@@ -2309,7 +2340,8 @@ int32_t fs_wcsph_adjoint_step(fs_grid* g, fs_work* w, const fs_model* m, const f
     fa.rhobar = rhobar;
     fa.grad_stiffness = io->grad_stiffness;
     fa.grad_viscosity = io->grad_viscosity;
-    LAUNCH_PASS(k_wadj_force<kReplayCap>, nb, s, fa);
+    const int nb_group = blocks_for(n, kPassBlock / kAdjLpp);
+    LAUNCH_PASS((k_wadj_force<kReplayCap, kAdjLpp>), nb_group, s, fa);
 
     WadjDensityArgs da;
     da.c = make_common(g, w, pos, h, kListUse);
@@ -2319,7 +2351,7 @@ int32_t fs_wcsph_adjoint_step(fs_grid* g, fs_work* w, const fs_model* m, const f
     da.vbar_s = vbar;
     da.xbar_in = io->xbar_in;
     da.vbar_in = io->vbar_in;
-    LAUNCH_PASS(k_wadj_density<kReplayCap>, nb, s, da);
+    LAUNCH_PASS((k_wadj_density<kReplayCap, kAdjLpp>), nb_group, s, da);
     return FS_OK;
 }
 
