@@ -75,11 +75,20 @@ SOLVER_BYTES = {"wcsph": 168, "pbf": 760}
 PMC_TRAFFIC = os.path.join(ROOT, "profiles", "wcsph_dam_break_401856_pmc_traffic.md")
 
 
-def pmc_traffic(kernel: str, block: int, depth: int):
-    """HBM bytes per launch of one launch variant of a kernel, from the
-    committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE runs scaled by
-    the copy calibration); None when the variant was not measured."""
-    key = f"{kernel}<{block}, {depth}, false>"
+def variant_key(kernel: str, v: dict) -> str:
+    """The kernel's name with the template arguments of a launch variant, as
+    rocprofv3 prints it."""
+    if kernel == "k_wcsph_density":
+        args = (v["block"], v["list_depth"], "false", v["lanes_per_particle"], v["unroll"])
+    else:
+        args = (v["block"], v["list_depth"], "false", v["unroll"], v["particles_per_thread"], v["lanes_per_particle"])
+    return f"{kernel}<{', '.join(str(a) for a in args)}>"
+
+
+def pmc_traffic(key: str):
+    """HBM bytes per launch of one launch variant of a kernel (variant_key),
+    from the committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE runs
+    scaled by the copy calibration); None when the variant was not measured."""
     try:
         with open(PMC_TRAFFIC) as f:
             for line in f:
@@ -96,11 +105,10 @@ PMC_SQ = os.path.join(ROOT, "profiles", "wcsph_dam_break_401856_pmc_sq.md")
 VALU_ISSUE_PEAK = 256 * 2.4e9
 
 
-def pmc_valu_instructions(kernel: str, block: int, depth: int):
-    """SQ_INSTS_VALU per launch of one launch variant (waves x VALU/wave), from
-    the committed rocprofv3 SQ summary; None when the variant was not
-    measured."""
-    key = f"{kernel}<{block}, {depth}, false>"
+def pmc_valu_instructions(key: str):
+    """SQ_INSTS_VALU per launch of one launch variant (variant_key; waves x
+    VALU/wave), from the committed rocprofv3 SQ summary; None when the
+    variant was not measured."""
     try:
         with open(PMC_SQ) as f:
             for line in f:
@@ -831,7 +839,7 @@ def main():
     tuned = headline.get("autotune", {})
     tkey = {"k_wcsph_density": "wcsph_density", "k_wcsph_force": "wcsph_force"}.get(dom)
     if tkey in tuned:
-        traffic = pmc_traffic(dom, tuned[tkey]["block"], tuned[tkey]["list_depth"])
+        traffic = pmc_traffic(variant_key(dom, tuned[tkey]))
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": avg_s * 1e6,
                 "bytes_per_launch": PASS_BYTES[dom] * scene.count,
@@ -839,7 +847,7 @@ def main():
                                   "selected launch variant: FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"}
     valu = None
     if tkey in tuned:
-        instr = pmc_valu_instructions(dom, tuned[tkey]["block"], tuned[tkey]["list_depth"])
+        instr = pmc_valu_instructions(variant_key(dom, tuned[tkey]))
         if instr:
             rate = instr / avg_s
             valu = {"bound": "valu_issue", "kernel": dom, "instructions_per_launch": instr, "achieved": rate,
