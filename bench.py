@@ -64,7 +64,7 @@ PUBLISHED_WCSPH = 62.78e6     # particle-substeps/s derived from the reference's
 # per-particle array it uses once and writes each result once; neighbour
 # gathers are on-chip reuse and cost nothing.
 PASS_BYTES = {
-    "k_wcsph_density": 44, "k_wcsph_force": 96,
+    "k_wcsph_density": 44, "k_wcsph_density_coop": 44, "k_wcsph_force": 96,
     "k_dfsph_density_factor": 40, "k_dfsph_nonpressure": 68, "k_dfsph_rate": 64, "k_dfsph_correction": 60,
     "k_pbf_density": 32, "k_pbf_nonpressure": 76, "k_pbf_lambda": 44, "k_pbf_delta": 72, "k_pbf_xsph": 80,
 }
@@ -78,7 +78,10 @@ PMC_TRAFFIC = os.path.join(ROOT, "profiles", "wcsph_dam_break_401856_pmc_traffic
 def variant_key(kernel: str, v: dict) -> str:
     """The kernel's name with the template arguments of a launch variant, as
     rocprofv3 prints it."""
-    if kernel == "k_wcsph_density":
+    if kernel == "k_wcsph_density" and v.get("coop"):
+        kernel = "k_wcsph_density_coop"
+        args = (v["list_depth"], v["coop"], "false", v["lanes_per_particle"], v["unroll"])
+    elif kernel.startswith("k_wcsph_density"):
         args = (v["block"], v["list_depth"], "false", v["lanes_per_particle"], v["unroll"])
     else:
         args = (v["block"], v["list_depth"], "false", v["unroll"], v["particles_per_thread"], v["lanes_per_particle"])
@@ -231,7 +234,7 @@ def bench_solver(method, scene, steps, warmup, tune=True, settle_frames=0):
         tuned = tune_passes(lambda: sim.solver.step(sim.state_0, s_scratch, None, None, dt), SOLVER_PASSES[method])
         print(f"# {method} autotune: " + ", ".join(
             f"{k} -> block {v['block']} depth {v['list_depth']} unroll {v['unroll']} ppt {v['particles_per_thread']}"
-            f" lpp {v['lanes_per_particle']}"
+            f" lpp {v['lanes_per_particle']}" + (f" coop {v['coop']}" if v.get("coop") else "")
             for k, v in tuned.items()), file=sys.stderr, flush=True)
     state = None
     if method == "wcsph":
@@ -267,7 +270,7 @@ def bench_solver(method, scene, steps, warmup, tune=True, settle_frames=0):
     if tuned:
         out["autotune"] = {k: {"block": v["block"], "list_depth": v["list_depth"], "unroll": v["unroll"],
                                "particles_per_thread": v["particles_per_thread"],
-                               "lanes_per_particle": v["lanes_per_particle"],
+                               "lanes_per_particle": v["lanes_per_particle"], "coop": v.get("coop", 0),
                                "us_per_step_by_variant": v["us_per_launch"]} for k, v in tuned.items()}
     if method == "dfsph":
         out["iterations_last_substep"] = iters
@@ -388,7 +391,7 @@ def micro(sizes, reps):
                     prof[key] = prof.get(key, 0.0) + ms
                 samples["build"].append(sum(prof[k] for k in ("k_hash_count", "k_bucket_scan", "k_scatter",
                                                                "k_fix_gather")))
-                samples["density"].append(prof["k_wcsph_density"])
+                samples["density"].append(sum(v for k, v in prof.items() if k.startswith("k_wcsph_density")))
                 samples["force"].append(prof["k_wcsph_force"])
             build, dens, force = (statistics.median(samples[k]) for k in ("build", "density", "force"))
             nbr = float(counts.float().mean().item()) - 1.0
@@ -494,7 +497,7 @@ def batched(num_scenes, steps, warmup, chunk=10):
                        "speedup_vs_sequential_graph": seq_graph / graphed,
                        "autotune": {k: {"block": v["block"], "list_depth": v["list_depth"], "unroll": v["unroll"],
                                         "particles_per_thread": v["particles_per_thread"],
-                                        "lanes_per_particle": v["lanes_per_particle"],
+                                        "lanes_per_particle": v["lanes_per_particle"], "coop": v.get("coop", 0),
                                         "us_per_launch_by_variant": v["us_per_launch"]} for k, v in tuned.items()}}}
     del graph, sims, batch
     torch.cuda.empty_cache()
@@ -837,7 +840,8 @@ def main():
     achieved = PASS_BYTES[dom] * scene.count / avg_s / 1e9
     traffic = None
     tuned = headline.get("autotune", {})
-    tkey = {"k_wcsph_density": "wcsph_density", "k_wcsph_force": "wcsph_force"}.get(dom)
+    tkey = {"k_wcsph_density": "wcsph_density", "k_wcsph_density_coop": "wcsph_density",
+            "k_wcsph_force": "wcsph_force"}.get(dom)
     if tkey in tuned:
         traffic = pmc_traffic(variant_key(dom, tuned[tkey]))
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -147,6 +147,7 @@ SIGNATURES = {
     "fs_tune_describe": (c_int32, [c_int32, c_int32, POINTER(c_int32), POINTER(c_int32)]),
     "fs_tune_describe_ex": (c_int32, [c_int32, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32),
                                       POINTER(c_int32), POINTER(c_int32)]),
+    "fs_tune_variant_coop": (c_int32, [c_int32, c_int32]),
     "fs_tune_select": (c_int32, [c_int32, c_int32]),
     "fs_tune_selected": (c_int32, [c_int32]),
     "fs_check_fast_math": (c_int32, [c_int32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,

@@ -24,8 +24,8 @@ from . import _lib
 PASSES = {"wcsph_density": 0, "wcsph_force": 1, "batch_density": 2, "batch_force": 3, "dfsph_scan": 4,
           "dfsph_replay": 5, "pbf_scan": 6, "pbf_replay": 7}
 # the kernels whose time a pass's variant changes
-KERNEL = {"wcsph_density": ("k_wcsph_density",), "wcsph_force": ("k_wcsph_force",),
-          "batch_density": ("k_wcsph_density",), "batch_force": ("k_wcsph_force",),
+KERNEL = {"wcsph_density": ("k_wcsph_density", "k_wcsph_density_coop"), "wcsph_force": ("k_wcsph_force",),
+          "batch_density": ("k_wcsph_density", "k_wcsph_density_coop"), "batch_force": ("k_wcsph_force",),
           "dfsph_scan": ("k_dfsph_density_factor",),
           "dfsph_replay": ("k_dfsph_nonpressure", "k_dfsph_rate", "k_dfsph_correction"),
           "pbf_scan": ("k_pbf_density", "k_pbf_lambda"),
@@ -36,8 +36,10 @@ SOLVER_PASSES = {"wcsph": ("wcsph_density", "wcsph_force"), "dfsph": ("dfsph_sca
 
 
 def describe(pass_name: str) -> list[dict]:
-    """Every variant of a pass: workgroup size, LDS list depth, replay unroll,
-    particles per lane and lanes per particle."""
+    """Every variant of a pass: workgroup size, LDS list depth (chunk slots of
+    a cooperative scan), replay unroll, particles per lane, lanes per
+    particle, and the cooperative scan's chunks per group (0: per-lane
+    scan)."""
     lib = _lib.load()
     pid = PASSES[pass_name]
     out = []
@@ -45,8 +47,11 @@ def describe(pass_name: str) -> list[dict]:
         b, d, u, p, l = (ctypes.c_int32() for _ in range(5))
         _lib.check(lib.fs_tune_describe_ex(pid, v, ctypes.byref(b), ctypes.byref(d), ctypes.byref(u),
                                            ctypes.byref(p), ctypes.byref(l)), "fs_tune_describe_ex")
+        coop = lib.fs_tune_variant_coop(pid, v)
+        if coop < 0:
+            _lib.check(coop, "fs_tune_variant_coop")
         out.append({"block": b.value, "list_depth": d.value, "unroll": u.value, "particles_per_thread": p.value,
-                    "lanes_per_particle": l.value})
+                    "lanes_per_particle": l.value, "coop": int(coop)})
     return out
 
 

@@ -15,6 +15,7 @@
 // and the same grid replays it instead of scanning the 27 buckets again.  A
 // particle's candidates depend on its current position (the query range is
 // recomputed from it), so every pass after the positions change scans again.
+#include "fs_coop.h"
 #include "fs_internal.h"
 
 #include <atomic>
@@ -444,10 +445,10 @@ struct WcsphDensityArgs {
 
 // compute_density_pressure  solver_wcsph.py:119-158, for slot s with the
 // pass parameters in `c` (LPP lanes per particle; the group's first lane
-// writes)
-template <int BLOCK, int CAP, int LPP, int U>
-__device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, const PassCommon& c, int clamp, int s,
-                                                   int* lds)
+// writes); loop(xi, load, term, fold) runs the neighbour loop
+template <int LPP, class Loop>
+__device__ __forceinline__ void wcsph_density_core(const WcsphDensityArgs& a, const PassCommon& c, int clamp, int s,
+                                                   Loop&& loop)
 {
     const bool lead = (threadIdx.x & (LPP - 1)) == 0;
     const int i = c.perm[s];
@@ -464,14 +465,14 @@ __device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, co
     }
     const Smooth sm = smooth_of(c.h);
     float rho_temp = 0.0f;
-    scan_pass<BLOCK, CAP, LPP, U>(
-        c, s, xi.x, xi.y, xi.z, lds, NoCount{}, [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.wvol[k]}; },
-        [&](int k, const NbrXS& nb) -> float {
-            if (!(nb.flags & kActive)) return 0.0f;
-            v3 dist = vsub(xyz(xi), xyz(nb.x));
-            return nb.s * cubic_w(vlen(dist), sm);
-        },
-        [&](float t) { rho_temp += t; });
+    auto load = [&](int k) { return NbrXS{c.sflags[k], c.pos[k], a.wvol[k]}; };
+    auto term = [&](int k, const NbrXS& nb) -> float {
+        if (!(nb.flags & kActive)) return 0.0f;
+        v3 dist = vsub(xyz(xi), xyz(nb.x));
+        return nb.s * cubic_w(vlen(dist), sm);
+    };
+    auto fold = [&](float t) { rho_temp += t; };
+    loop(xi, load, term, fold);
     if (!lead) return;
     const float rho_i = rho_temp;
     float p = a.stiffness[i] * (fpow_(rho_i / fmax_(a.rest_density[i], 1.0e-6f), a.exponent[i]) - 1.0f);
@@ -480,6 +481,22 @@ __device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, co
     a.pressure[i] = p;
     a.sca[s] = make_float4(rho_i, p, p / fmax_(rho_i * rho_i, 1.0e-6f), 0.0f);
     a.vel[s].w = xi.w / fmax_(rho_i, 1.0e-6f);
+}
+
+template <int BLOCK, int CAP, int LPP, int U>
+__device__ __forceinline__ void wcsph_density_lane(const WcsphDensityArgs& a, const PassCommon& c, int clamp, int s,
+                                                   int* lds)
+{
+    wcsph_density_core<LPP>(a, c, clamp, s, [&](const float4& xi, auto& load, auto& term, auto& fold) {
+        scan_pass<BLOCK, CAP, LPP, U>(c, s, xi.x, xi.y, xi.z, lds, NoCount{}, load, term, fold);
+    });
+}
+
+// The keep radius of a scanning pass: the list's record radius when it
+// records, else the pass cut (as in scan_neighbors / scan_split).
+__device__ __forceinline__ float scan_keep2(const PassCommon& c)
+{
+    return c.L.mode == kListBuild ? fmax_(c.L.cut2, c.cut2) : c.cut2;
 }
 
 // Batched passes: slots are sorted by bucket and every scene owns a
@@ -525,6 +542,67 @@ __global__ __launch_bounds__(BLOCK) void k_wcsph_density(WcsphDensityArgs a)
             return;
         }
         wcsph_density_lane<BLOCK, CAP, LPP, U>(a, a.c, a.clamp, s, lds);
+    }
+}
+
+// The density/EOS pass with the cooperative cell scan (fs_coop.h): a
+// workgroup of 64 LPP threads per tile of 64 slots, NCH chunk slots of
+// staged candidates and up to MAXC chunks per group.
+template <int NCH, int MAXC, bool BATCH, int LPP, int U>
+__global__ __launch_bounds__(kCoopTile * LPP) void k_wcsph_density_coop(WcsphDensityArgs a)
+{
+    __shared__ CoopShared<NCH, MAXC> sh;
+    const int s0 = xcd_tile() * kCoopTile;
+    if constexpr (!BATCH) {
+        if (a.dt_device && *a.dt_device == 0.0f) {   // idle substep: carry rho and p over
+            const int s = s0 + (int)threadIdx.x;
+            if (threadIdx.x < kCoopTile && s < a.c.n) {
+                if (a.rho_in) a.rho[s] = a.rho_in[s];
+                if (a.pressure_in) a.pressure[s] = a.pressure_in[s];
+            }
+            return;
+        }
+    }
+    const int tl = s0 + (int)(threadIdx.x & 63);
+    const int sid = (BATCH && tl < a.c.n) ? a.sscene[tl] : 0;
+    coop_test<NCH, MAXC, LPP>(a.c.pos, a.c.sflags, a.c.n, s0, sid,
+                              [&](int id, GridView& g, float& r, float& keep2) -> bool {
+                                  if constexpr (BATCH) {
+                                      const fs_scene& sc = a.scenes[id];
+                                      if (sc.dt == 0.0f) return false;
+                                      PassCommon c = a.c;
+                                      scene_view(c, sc);
+                                      g = c.g;
+                                      r = c.h;
+                                      keep2 = scan_keep2(c);
+                                  } else {
+                                      g = a.c.g;
+                                      r = a.c.h;
+                                      keep2 = scan_keep2(a.c);
+                                  }
+                                  return true;
+                              },
+                              sh);
+    const int p = (int)(threadIdx.x / LPP);
+    const int s = s0 + p;
+    if (s >= a.c.n) return;
+    auto run = [&](const PassCommon& c, int clamp) {
+        const float keep2 = scan_keep2(c);
+        wcsph_density_core<LPP>(a, c, clamp, s, [&](const float4& xi, auto& load, auto& term, auto& fold) {
+            coop_neighbors<NCH, MAXC, LPP, U>(sh, c.g, c.pos, p, s, xi.x, xi.y, xi.z, c.h, keep2, c.L, load, term,
+                                              fold);
+        });
+    };
+    if constexpr (BATCH) {
+        FOR_MY_SCENE(a, s, sc, {
+            if (sc.dt != 0.0f) {
+                PassCommon c = a.c;
+                scene_view(c, sc);
+                run(c, sc.clamp_negative_pressure);
+            }
+        });
+    } else {
+        run(a.c, a.clamp);
     }
 }
 
@@ -1836,6 +1914,7 @@ struct Variant {
     int unroll;    // neighbours whose data a list replay fetches together (per lane)
     int ppt;       // particles per lane
     int lpp = 1;   // lanes per particle (block / lpp >= 64 particles per workgroup)
+    int coop = 0;  // cooperative cell scan (fs_coop.h): chunks per group (cap = chunk slots per tile); 0: per-lane
 };
 // variant 0 of each table is the default: the fastest at the developed
 // 401,856-particle dam break
@@ -1843,7 +1922,11 @@ constexpr Variant kScanVariants[] = {{128, 48, 4, 1, 2}, {64, 24, 4, 1},     {64
                                      {128, 24, 4, 1},    {128, 32, 4, 1},    {128, 16, 4, 1},    {256, 32, 4, 1},
                                      {256, 24, 4, 1},    {256, 16, 4, 1},    {64, 40, 4, 1},     {64, 56, 4, 1},
                                      {64, 48, 4, 1},     {128, 32, 4, 1, 2}, {128, 64, 2, 1, 2}, {256, 48, 2, 1, 4},
-                                     {256, 32, 2, 1, 4}};
+                                     {256, 32, 2, 1, 4},
+                                     // cooperative cell scan (fs_coop.h; measured slower at the dam
+                                     // break, kept for the autotuner): 64 lpp threads, cap chunk slots
+                                     // per tile, coop chunks per group
+                                     {128, 48, 4, 1, 2, 8},   {256, 48, 2, 1, 4, 8}};
 constexpr Variant kReplayVariants[] = {{128, 16, 4, 1, 2}, {128, 16, 4, 1},    {256, 16, 4, 1},    {512, 16, 4, 1},
                                        {64, 16, 8, 1},     {256, 16, 8, 1},    {64, 16, 4, 2},     {256, 16, 4, 2},
                                        {256, 16, 4, 1, 4}, {256, 16, 2, 1, 4}, {64, 16, 4, 1},     {256, 16, 4, 1, 2}};
@@ -1928,8 +2011,13 @@ template <bool BATCH>
 static int32_t launch_density(int v, int n, hipStream_t s, const WcsphDensityArgs& a)
 {
     return with_variant<kTabScan>(v, [&]<Variant V>() -> int32_t {
-        FS_LAUNCH((k_wcsph_density<V.block, V.cap, BATCH, V.lpp, V.unroll>), dim3(blocks_for(n, V.block / V.lpp)),
-                  dim3(V.block), 0, s, a);
+        if constexpr (V.coop > 0) {
+            FS_LAUNCH((k_wcsph_density_coop<V.cap, V.coop, BATCH, V.lpp, V.unroll>),
+                      dim3(blocks_for(n, kCoopTile)), dim3(kCoopTile * V.lpp), 0, s, a);
+        } else {
+            FS_LAUNCH((k_wcsph_density<V.block, V.cap, BATCH, V.lpp, V.unroll>), dim3(blocks_for(n, V.block / V.lpp)),
+                      dim3(V.block), 0, s, a);
+        }
         return FS_OK;
     });
 }
@@ -2209,6 +2297,15 @@ int32_t fs_tune_describe_ex(int32_t pass, int32_t variant, int32_t* block, int32
     if (particles_per_thread) *particles_per_thread = v.ppt;
     if (lanes_per_particle) *lanes_per_particle = v.lpp;
     return FS_OK;
+}
+
+int32_t fs_tune_variant_coop(int32_t pass, int32_t variant)
+{
+    if (pass < 0 || pass >= kTunePasses || variant < 0 || variant >= tune_count(pass)) {
+        set_error("fs_tune_variant_coop: unknown pass %d / variant %d", pass, variant);
+        return FS_ERR_ARG;
+    }
+    return variant_at(kTuneTable[pass], variant).coop;
 }
 
 int32_t fs_tune_select(int32_t pass, int32_t variant)

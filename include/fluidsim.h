@@ -83,6 +83,11 @@ int32_t fs_tune_describe(int32_t pass, int32_t variant, int32_t* block, int32_t*
  * per lane and lanes per particle of a variant (any pointer may be NULL) */
 int32_t fs_tune_describe_ex(int32_t pass, int32_t variant, int32_t* block, int32_t* list_depth, int32_t* unroll,
                             int32_t* particles_per_thread, int32_t* lanes_per_particle);
+/* ... and its scan kind: 0 for the per-lane scan, else the cooperative cell
+ * scan (candidates staged per query group in LDS and tested against every
+ * particle of the group at once) with this many 64-candidate chunks per
+ * group; list_depth then counts the tile's chunk slots.  Negative on error. */
+int32_t fs_tune_variant_coop(int32_t pass, int32_t variant);
 int32_t fs_tune_select(int32_t pass, int32_t variant);
 int32_t fs_tune_selected(int32_t pass);
 
