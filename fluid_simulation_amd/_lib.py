@@ -40,6 +40,9 @@ class FsModel(ctypes.Structure):
     ]
 
 
+FS_CONTROL_LIST_NEED = 7   # fs_work::control word: longest overflowed list needed
+
+
 class FsWork(ctypes.Structure):
     _fields_ = [
         ("capacity", c_int32), ("list_capacity", c_int32),

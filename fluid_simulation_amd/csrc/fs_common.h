@@ -484,7 +484,21 @@ struct NbrList {
     int n;
     int mode;      // kListNone: scan only; kListBuild: scan and record; kListUse: replay
     float cut2;    // squared record radius (>= the pass cut; larger for a skin)
+    int* need;     // optional: atomic max of the lengths of lists that overflowed
 };
+
+// Length of slot s's recorded list (cap + 1 marks an overflowed list; the
+// length it needed then goes to L.need, which the host reads to grow the
+// lists).
+__device__ __forceinline__ void store_list_count(const NbrList& L, int s, int written)
+{
+    if (written <= L.cap) {
+        L.cnt[s] = written;
+    } else {
+        L.cnt[s] = L.cap + 1;
+        if (L.need) atomicMax(L.need, written);
+    }
+}
 constexpr int kListNone = 0;
 constexpr int kListBuild = 1;
 constexpr int kListUse = 2;
@@ -690,7 +704,7 @@ __device__ __forceinline__ int scan_neighbors(const GridView& g, const float4* _
             if (done) break;
         }
     }
-    if (record) L.cnt[s] = written <= L.cap ? written : L.cap + 1;
+    if (record) store_list_count(L, s, written);
     return candidates;
 }
 
@@ -1022,7 +1036,7 @@ __device__ __forceinline__ int scan_split(const GridView& g, const float4* __res
             if (done) break;
         }
     }
-    if (record && q == 0) L.cnt[s] = written <= L.cap ? written : L.cap + 1;
+    if (record && q == 0) store_list_count(L, s, written);
     return candidates;
 }
 

@@ -344,7 +344,7 @@ __device__ __forceinline__ void coop_walk(const CoopShared<NCH, MAXC>& sh, int p
         }
         done = last < 0;
     }
-    if (record && q == 0) L.cnt[s] = written <= L.cap ? written : L.cap + 1;
+    if (record && q == 0) store_list_count(L, s, written);
 }
 
 // One lane's scan of hash_grid_query(p, r) with no LDS: every candidate in
@@ -379,7 +379,7 @@ __device__ __forceinline__ void scan_direct(const GridView& g, const float4* __r
             }
         } while (it.advance(g));
     }
-    if (record) L.cnt[s] = written <= L.cap ? written : L.cap + 1;
+    if (record) store_list_count(L, s, written);
 }
 
 // Neighbour loop of particle p of the tile (slot s) after coop_test: the

@@ -39,6 +39,7 @@ enum Control : int {
     kItVel = 4,
     kCorrVel = 5,
     kAnyInactive = 6,   // set by the builds when some particle is inactive
+    kListNeed = FS_CONTROL_LIST_NEED,   // the longest list an overflowing particle needed
     kControlWords = 64,
 };
 
@@ -2045,6 +2046,7 @@ static NbrList list_of(const fs_work* w, int n, int mode, float cut2)
     L.n = n;
     L.mode = (w->list_capacity > 0 && w->nbr && w->nbr_count) ? mode : kListNone;
     L.cut2 = cut2;
+    L.need = w->control ? w->control + kListNeed : nullptr;
     return L;
 }
 

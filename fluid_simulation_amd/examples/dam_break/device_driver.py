@@ -17,6 +17,10 @@ the host before every WCSPH/DFSPH substep.  Here:
   and the ``sph`` arrays) to the output.  Its substeps are long and few per
   frame, so its chunks are short (4 substeps).
 * PBF has a fixed substep (frame/substeps), so a whole frame is one graph.
+
+Before each frame the driver lets the solver grow its neighbour lists if the
+last replayed substep overflowed them (``adapt_lists``), and captures the
+graph again when it did: the graph holds the workspace's addresses.
 """
 from __future__ import annotations
 
@@ -132,6 +136,8 @@ class DeviceResidentDamBreak:
         """Advance one display frame; returns the substeps run (WCSPH: the
         substeps replayed, including idle ones after the frame completed)."""
         sim = self.sim
+        if sim.solver.adapt_lists():
+            self._graph = None                 # the graph holds the old workspace's lists
         if self.method == "pbf":
             if self.substeps % 2:
                 self._pbf_frame()              # odd substep counts: eager frames

@@ -112,3 +112,4 @@ class SolverPBF(ParticleSolver):
         _lib.check(_lib.load().fs_pbf_step(ctypes.byref(self.model.particle_grid._c), ctypes.byref(self._work),
                                            ctypes.byref(m), ctypes.byref(p), ctypes.byref(io),
                                            ctypes.c_void_p(self._stream())), "fs_pbf_step")
+        self._lists_after_step()

@@ -131,3 +131,4 @@ class SolverDFSPH(ParticleSolver):
         _lib.check(_lib.load().fs_dfsph_step(ctypes.byref(self.model.particle_grid._c), ctypes.byref(self._work),
                                              ctypes.byref(m), ctypes.byref(p), ctypes.byref(io),
                                              ctypes.c_void_p(self._stream())), "fs_dfsph_step")
+        self._lists_after_step()

@@ -82,3 +82,4 @@ class SolverWCSPH(ParticleSolver):
         _lib.check(lib.fs_wcsph_step(ctypes.byref(self.model.particle_grid._c), ctypes.byref(self._work),
                                      ctypes.byref(m), ctypes.byref(p), ctypes.byref(io),
                                      ctypes.c_void_p(self._stream())), "fs_wcsph_step")
+        self._lists_after_step()

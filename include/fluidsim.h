@@ -211,7 +211,8 @@ typedef struct fs_work {
     int32_t* nbr;       /* [list_capacity*cap] neighbour lists (entry-major)    */
     int32_t* nbr_count; /* [cap] list lengths (list_capacity+1 = overflowed)    */
     double* partials;   /* residual partial sums                               */
-    int32_t* control;   /* [64] loop control: stop flags, iteration counters   */
+    int32_t* control;   /* [64] loop control: stop flags, iteration counters;
+                           word FS_CONTROL_LIST_NEED: see below            */
     uint32_t* sync;     /* [8]  self-resetting counters                        */
     int32_t* packed;    /* [cap*4] grid-build scratch, original order: mass,
                            rest density, flags, mask of each particle packed
@@ -223,6 +224,12 @@ typedef struct fs_work {
  * same candidates (0 disables them; every pass then scans the grid).  The
  * lists take capacity * list_capacity * 4 bytes; capacity <= FS_MAX_PARTICLES. */
 size_t fs_work_bytes(int32_t capacity, int32_t list_capacity);
+
+/* control[FS_CONTROL_LIST_NEED]: the longest list a particle needed whose list
+ * overflowed list_capacity (0 while none has).  fs_work_init clears it, and
+ * so does every fs_dfsph_step (its first grid build clears the loop-control
+ * words); the host reads it to re-initialise a workspace with longer lists. */
+#define FS_CONTROL_LIST_NEED 7
 int32_t fs_work_init(fs_work* work, void* workspace, size_t workspace_bytes, int32_t capacity,
                      int32_t list_capacity, fs_stream_t stream);
 
