@@ -205,7 +205,7 @@ __device__ __forceinline__ void coop_test(const float4* __restrict__ pos, const 
         float cx[MAXC], cy[MAXC], cz[MAXC], cc[MAXC];
         int slot[MAXC];
 #pragma unroll
-        for (int c = 0; c < MAXC; ++c) slot[c] = c < nch ? sh.slots[(cfirst + c) * 64 + lane] : 0;
+        for (int c = 0; c < MAXC; ++c) slot[c] = (c < nch && 64 * c + lane < total) ? sh.slots[(cfirst + c) * 64 + lane] : 0;
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) {
             if (c < nch) {
