@@ -520,6 +520,13 @@ __device__ __forceinline__ float4 ld_f4(__amdgpu_buffer_rsrc_t r, int base_bytes
 {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, base_bytes, imm, 0));
 }
+__device__ __forceinline__ float ld_f32(__amdgpu_buffer_rsrc_t r, int base_bytes, int imm)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, base_bytes, imm, 0));
+}
+// Byte offset of slot k in a sorted float4 stream (one 32-bit offset serves
+// every stream a neighbour body reads; slot counts are capped so it fits).
+__device__ __forceinline__ int slot_off(int k) { return (int)((unsigned)k * 16u); }
 
 // Squared distance for the survivor prefilter only: fused multiply-adds (3
 // instructions instead of 5).  It differs from the reference's written-order

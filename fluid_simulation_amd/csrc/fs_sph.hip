@@ -679,9 +679,14 @@ __device__ __forceinline__ v3 wcsph_force_sum(const WcsphForceArgs& a, const Pas
     const float visc_den0 = 0.01f * (h * h);
     const float mu10 = 10.0f * mu;
     v3 acc = V(0.0f, 0.0f, 0.0f);
+    const __amdgpu_buffer_rsrc_t rp = float4_rsrc(c.pos, c.n), rv = float4_rsrc(a.vel, c.n),
+                                 rsc = float4_rsrc(a.sca, c.n);
     for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
         c.g, c.pos, s, xi.x, xi.y, xi.z, c.h, c.cut2, lds, c.L,
-        [&](int k) { return NbrXVS{c.sflags[k], c.pos[k], a.vel[k], a.sca[k].z}; },
+        [&](int k) {
+            const int o = slot_off(k);
+            return NbrXVS{c.sflags[k], ld_f4(rp, o, 0), ld_f4(rv, o, 0), ld_f32(rsc, o, 8)};
+        },
         [&](int k, const NbrXVS& nb) -> ForceTerms<COH> {
             ForceTerms<COH> t;
             if constexpr (COH) t.coh = V(0.0f, 0.0f, 0.0f);
@@ -1295,12 +1300,16 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_rate(DfRateArgs a)
             const v3 vi = xyz(a.vstar_s[s]);
             const Smooth sm = smooth_of(a.c.h);
             float div_val = 0.0f;
+            const __amdgpu_buffer_rsrc_t rp = float4_rsrc(a.c.pos, a.c.n), rv = float4_rsrc(a.vstar_s, a.c.n);
             // a skipped (inactive) neighbour contributes +0: div_val starts at
             // +0 and a round-to-nearest sum from +0 never becomes -0, so adding
             // +0 leaves it unchanged
             for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
                 a.c.g, a.c.pos, s, xi.x, xi.y, xi.z, a.c.h, a.c.cut2, lds, a.c.L,
-                [&](int k) { return NbrXV{a.c.sflags[k], a.c.pos[k], a.vstar_s[k]}; },
+                [&](int k) {
+                    const int o = slot_off(k);
+                    return NbrXV{a.c.sflags[k], ld_f4(rp, o, 0), ld_f4(rv, o, 0)};
+                },
                 [&](int k, const NbrXV& nb) -> float {
                     if (!(nb.flags & kActive)) return 0.0f;
                     const float4 xj = nb.x;
@@ -1373,10 +1382,14 @@ __global__ __launch_bounds__(BLOCK) void k_dfsph_correction(DfCorrArgs a)
     const Smooth sm = smooth_of(a.c.h);
     const float kri = si.w;
     v3 dv = V(0.0f, 0.0f, 0.0f);
+    const __amdgpu_buffer_rsrc_t rp = float4_rsrc(a.c.pos, a.c.n), rsc = float4_rsrc(a.sca, a.c.n);
     // skipped neighbours contribute +0 (see k_dfsph_rate)
     for_each_neighbor_split<BLOCK, CAP, UNR, LPP>(
         a.c.g, a.c.pos, s, xi.x, xi.y, xi.z, a.c.h, a.c.cut2, lds, a.c.L,
-        [&](int k) { return NbrXS{a.c.sflags[k], a.c.pos[k], a.sca[k].w}; },
+        [&](int k) {
+            const int o = slot_off(k);
+            return NbrXS{a.c.sflags[k], ld_f4(rp, o, 0), ld_f32(rsc, o, 12)};
+        },
         [&](int k, const NbrXS& nb) -> v3 {
             if (!(nb.flags & kActive)) return V(0.0f, 0.0f, 0.0f);
             const float4 xj = nb.x;
