@@ -356,10 +356,13 @@ def copy_calibration():
 def micro(sizes, reps):
     """Grid build, density pass and force pass at 4M / 16M points, uniform and
     clustered (per-kernel HIP-event timing; median over ``reps`` >= 20
-    substeps after a warm-up substep)."""
+    substeps after a warm-up substep), with the default launch variants and
+    with the variants the autotuner picks on each input."""
     import torch
 
     from fluid_simulation_amd import _lib
+    from fluid_simulation_amd.autotune import select as select_variant
+    from fluid_simulation_amd.autotune import tune as tune_passes
     from fluid_simulation_amd.examples import microbench as mb
     from fluid_simulation_amd.solver.sph.solver_wcsph import SolverWCSPH
 
@@ -379,25 +382,39 @@ def micro(sizes, reps):
             solver = SolverWCSPH(model, cfg, neighbor_list_capacity=cap)
             solver.step(s0, s1, None, None, 1e-5)          # warm-up
             torch.cuda.synchronize()
-            samples = {"build": [], "density": [], "force": []}
-            for _ in range(max(20, reps)):
-                lib.fs_profile_reset()
-                lib.fs_profile_enable(1)
-                solver.step(s0, s1, None, None, 1e-5)
-                lib.fs_profile_enable(0)
-                prof = {}
-                for name, (_cnt, ms) in _lib.profile_report().items():
-                    key = re.sub(r"<.*>", "", name.strip("()"))
-                    prof[key] = prof.get(key, 0.0) + ms
-                samples["build"].append(sum(prof[k] for k in ("k_hash_count", "k_bucket_scan", "k_scatter",
-                                                               "k_fix_gather")))
-                samples["density"].append(sum(v for k, v in prof.items() if k.startswith("k_wcsph_density")))
-                samples["force"].append(prof["k_wcsph_force"])
-            build, dens, force = (statistics.median(samples[k]) for k in ("build", "density", "force"))
+
+            def sample():
+                samples = {"build": [], "density": [], "force": []}
+                for _ in range(max(20, reps)):
+                    lib.fs_profile_reset()
+                    lib.fs_profile_enable(1)
+                    solver.step(s0, s1, None, None, 1e-5)
+                    lib.fs_profile_enable(0)
+                    prof = {}
+                    for name, (_cnt, ms) in _lib.profile_report().items():
+                        key = re.sub(r"<.*>", "", name.strip("()"))
+                        prof[key] = prof.get(key, 0.0) + ms
+                    samples["build"].append(sum(prof[k] for k in ("k_hash_count", "k_bucket_scan", "k_scatter",
+                                                                   "k_fix_gather")))
+                    samples["density"].append(sum(v for k, v in prof.items() if k.startswith("k_wcsph_density")))
+                    samples["force"].append(prof["k_wcsph_force"])
+                return tuple(statistics.median(samples[k]) for k in ("build", "density", "force"))
+
+            build, dens0, force0 = sample()                  # the default launch variants
+            tuned = tune_passes(lambda: solver.step(s0, s1, None, None, 1e-5), ("wcsph_density", "wcsph_force"))
+            build, dens, force = sample()                    # the variants the tuner picked on this input
+            for name in ("wcsph_density", "wcsph_force"):
+                select_variant(name, 0)
             nbr = float(counts.float().mean().item()) - 1.0
             print(json.dumps({"micro": dist, "particles": n, "h": solver.smoothing_length, "mean_neighbors": nbr,
                               "max_neighbors": int(counts.max().item()) - 1, "list_capacity": cap,
                               "grid_build_ms": build, "density_pass_ms": dens, "force_pass_ms": force,
+                              "density_pass_ms_default_variant": dens0, "force_pass_ms_default_variant": force0,
+                              "autotune": {k: {**{f: v[f] for f in ("block", "list_depth", "unroll",
+                                                                     "particles_per_thread", "lanes_per_particle",
+                                                                     "coop")},
+                                               "us_per_launch_by_variant": [round(t, 1) for t in v["us_per_launch"]]}
+                                           for k, v in tuned.items()},
                               "density_GBs_alg": PASS_BYTES["k_wcsph_density"] * n / dens / 1e6,
                               "force_GBs_alg": PASS_BYTES["k_wcsph_force"] * n / force / 1e6}), flush=True)
             del solver, model, s0, s1

@@ -46,12 +46,14 @@
 namespace fs {
 
 constexpr int kCoopTile = 64;            // particles per workgroup tile
+constexpr int kCoopBatch = 8;            // chunks whose candidates a wave holds in registers at once
 constexpr float kCoopMargin = 1.0e-3f;   // relative widening of the superset test
 
 template <int NCH, int MAXC>
 struct CoopShared {
     int slots[NCH * kCoopTile];              // chunk c's candidate slots at [c*64, c*64+64)
     uint32_t masks[2 * MAXC * kCoopTile];    // word w (= 2j + half) of chunk j of particle p at [w*64 + p]
+                                             // (MAXC: chunks per group)
     int2 ginfo[kCoopTile];                   // per particle: first chunk, mask words; words -1: scan alone
 };
 
@@ -200,59 +202,66 @@ __device__ __forceinline__ void coop_test(const float4* __restrict__ pos, const 
         const int nch = (total + 63) >> 6;
         cnext = cfirst + nch;
         wave_lds_sync();
-        // every chunk's candidates at once: c' = c - o and |c'|^2
+        // the chunks' candidates kCoopBatch chunks at a time: c' = c - o and
+        // |c'|^2, tested against every member
         const __amdgpu_buffer_rsrc_t rs = float4_rsrc(pos, g.npts);
-        float cx[MAXC], cy[MAXC], cz[MAXC], cc[MAXC];
-        int slot[MAXC];
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c) slot[c] = (c < nch && 64 * c + lane < total) ? sh.slots[(cfirst + c) * 64 + lane] : 0;
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c) {
-            if (c < nch) {
-                const float4 q = ld_f4(rs, (int)((unsigned)slot[c] * 16u), 0);
-                cx[c] = q.x - ox;
-                cy[c] = q.y - oy;
-                cz[c] = q.z - oz;
-                cc[c] = __builtin_fmaf(cz[c], cz[c], __builtin_fmaf(cy[c], cy[c], cx[c] * cx[c]));
-            }
-        }
         // members: -2p' and the threshold keep2' - |p'|^2
         const float nax = -2.0f * ax, nay = -2.0f * ay, naz = -2.0f * az;
         const float thr = __builtin_fmaf(-az, az, __builtin_fmaf(-ay, ay, __builtin_fmaf(-ax, ax, keep2c)));
-        int mlo[MAXC], mhi[MAXC];
+        for (int cb = 0; cb < nch; cb += kCoopBatch) {
+            const int nb = min(kCoopBatch, nch - cb);
+            float cx[kCoopBatch], cy[kCoopBatch], cz[kCoopBatch], cc[kCoopBatch];
+            int slot[kCoopBatch];
 #pragma unroll
-        for (int c = 0; c < MAXC; ++c) mlo[c] = mhi[c] = 0;
-        uint64_t pm = gm;
-        while (pm) {
-            const int p = (int)__builtin_ctzll(pm);
-            pm &= pm - 1ull;
-            const float px = readlane_f(nax, p), py = readlane_f(nay, p), pz = readlane_f(naz, p);
-            const float pt = readlane_f(thr, p);
-            const bool me = lane == p;
+            for (int c = 0; c < kCoopBatch; ++c)
+                slot[c] = (c < nb && 64 * (cb + c) + lane < total) ? sh.slots[(cfirst + cb + c) * 64 + lane] : 0;
 #pragma unroll
-            for (int c = 0; c < MAXC; ++c) {
-                if (c < nch) {
-                    float t = __builtin_fmaf(px, cx[c], cc[c]);
-                    t = __builtin_fmaf(py, cy[c], t);
-                    t = __builtin_fmaf(pz, cz[c], t);
-                    const int left = total - 64 * c;
-                    const uint64_t vm = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
-                    const uint64_t hm = __ballot(!(t > pt)) & vm;
-                    if (me) {
-                        mlo[c] = (int)(uint32_t)hm;
-                        mhi[c] = (int)(uint32_t)(hm >> 32);
+            for (int c = 0; c < kCoopBatch; ++c) {
+                if (c < nb) {
+                    const float4 q = ld_f4(rs, (int)((unsigned)slot[c] * 16u), 0);
+                    cx[c] = q.x - ox;
+                    cy[c] = q.y - oy;
+                    cz[c] = q.z - oz;
+                    cc[c] = __builtin_fmaf(cz[c], cz[c], __builtin_fmaf(cy[c], cy[c], cx[c] * cx[c]));
+                }
+            }
+            int mlo[kCoopBatch], mhi[kCoopBatch];
+#pragma unroll
+            for (int c = 0; c < kCoopBatch; ++c) mlo[c] = mhi[c] = 0;
+            uint64_t pm = gm;
+            while (pm) {
+                const int p = (int)__builtin_ctzll(pm);
+                pm &= pm - 1ull;
+                const float px = readlane_f(nax, p), py = readlane_f(nay, p), pz = readlane_f(naz, p);
+                const float pt = readlane_f(thr, p);
+                const bool me = lane == p;
+#pragma unroll
+                for (int c = 0; c < kCoopBatch; ++c) {
+                    if (c < nb) {
+                        float t = __builtin_fmaf(px, cx[c], cc[c]);
+                        t = __builtin_fmaf(py, cy[c], t);
+                        t = __builtin_fmaf(pz, cz[c], t);
+                        const int left = total - 64 * (cb + c);
+                        const uint64_t vm = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
+                        const uint64_t hm = __ballot(!(t > pt)) & vm;
+                        if (me) {
+                            mlo[c] = (int)(uint32_t)hm;
+                            mhi[c] = (int)(uint32_t)(hm >> 32);
+                        }
+                    }
+                }
+            }
+            if (member) {
+#pragma unroll
+                for (int c = 0; c < kCoopBatch; ++c) {
+                    if (c < nb) {
+                        sh.masks[(2 * (cb + c)) * kCoopTile + lane] = (uint32_t)mlo[c];
+                        sh.masks[(2 * (cb + c) + 1) * kCoopTile + lane] = (uint32_t)mhi[c];
                     }
                 }
             }
         }
         if (member) {
-#pragma unroll
-            for (int c = 0; c < MAXC; ++c) {
-                if (c < nch) {
-                    sh.masks[(2 * c) * kCoopTile + lane] = (uint32_t)mlo[c];
-                    sh.masks[(2 * c + 1) * kCoopTile + lane] = (uint32_t)mhi[c];
-                }
-            }
             sh.ginfo[lane] = make_int2(cfirst, 2 * nch);
         }
     }

@@ -1958,10 +1958,11 @@ constexpr Variant kScanVariants[] = {{128, 48, 4, 1, 2}, {64, 24, 4, 1},     {64
                                      {256, 24, 4, 1},    {256, 16, 4, 1},    {64, 40, 4, 1},     {64, 56, 4, 1},
                                      {64, 48, 4, 1},     {128, 32, 4, 1, 2}, {128, 64, 2, 1, 2}, {256, 48, 2, 1, 4},
                                      {256, 32, 2, 1, 4},
-                                     // cooperative cell scan (fs_coop.h; measured slower at the dam
-                                     // break, kept for the autotuner): 64 lpp threads, cap chunk slots
-                                     // per tile, coop chunks per group
-                                     {128, 48, 4, 1, 2, 8},   {256, 48, 2, 1, 4, 8}};
+                                     // cooperative cell scan (fs_coop.h): 64 lpp threads, cap chunk
+                                     // slots per tile, coop chunks per group (the second holds the
+                                     // 2,048-candidate queries of dense inputs).  Measured slower than
+                                     // the per-lane scan at the dam break and on clustered points
+                                     {128, 48, 4, 1, 2, 8},   {128, 64, 4, 1, 2, 32}};
 constexpr Variant kReplayVariants[] = {{128, 16, 4, 1, 2}, {128, 16, 4, 1},    {256, 16, 4, 1},    {512, 16, 4, 1},
                                        {64, 16, 8, 1},     {256, 16, 8, 1},    {64, 16, 4, 2},     {256, 16, 4, 2},
                                        {256, 16, 4, 1, 4}, {256, 16, 2, 1, 4}, {64, 16, 4, 1},     {256, 16, 4, 1, 2}};
