@@ -427,7 +427,8 @@ def batched(num_scenes, steps, warmup, chunk=10):
     a HIP graph of ``chunk`` substeps; (b) the same eagerly; (c) 64 sequential
     plain ``SolverWCSPH.step`` calls per substep, one per scene, with each
     scene's dt taken from the batch; (d) those 64 plain steps captured in one
-    graph.  Every variant advances every scene by one substep per step."""
+    graph; (e) the batched substeps of (a) without the CFL, the work of (c)
+    and (d).  Every variant advances every scene by one substep per step."""
     import torch
 
     from fluid_simulation_amd.batched import BatchedWCSPH, make_batch_scenes
@@ -472,6 +473,25 @@ def batched(num_scenes, steps, warmup, chunk=10):
     dts = batch.dts()
     del graph
 
+    # the same substeps without the on-device CFL (each scene keeps its last
+    # dt), the work the sequential plain solves below do: one graph of
+    # ``chunk`` advances
+    graph = torch.cuda.CUDAGraph()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(graph, stream=side):
+            for _ in range(chunk):
+                batch.advance()
+    torch.cuda.current_stream().wait_stream(side)
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps // chunk):
+        graph.replay()
+    torch.cuda.synchronize()
+    graphed_nocfl = (time.perf_counter() - t0) / steps
+    del graph
+
     sims = [DamBreakSimulation(s, "wcsph", device="cuda:0") for s in scenes]
     for _ in range(2):
         for sim, dt in zip(sims, dts):
@@ -512,6 +532,8 @@ def batched(num_scenes, steps, warmup, chunk=10):
                        "particle_substeps_per_s_sequential_eager": total / seq_eager,
                        "speedup_vs_sequential_eager": seq_eager / graphed,
                        "speedup_vs_sequential_graph": seq_graph / graphed,
+                       "ms_per_substep_graph_without_cfl": graphed_nocfl * 1e3,
+                       "speedup_same_work_vs_sequential_graph": seq_graph / graphed_nocfl,
                        "autotune": {k: {"block": v["block"], "list_depth": v["list_depth"], "unroll": v["unroll"],
                                         "particles_per_thread": v["particles_per_thread"],
                                         "lanes_per_particle": v["lanes_per_particle"], "coop": v.get("coop", 0),

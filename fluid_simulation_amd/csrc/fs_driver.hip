@@ -16,6 +16,7 @@ static __device__ __align__(256) uint32_t scratch[2 * kNoAllocMaxScenes];
 namespace fs {
 
 constexpr int kRedBlock = 256;
+constexpr int kRedUnroll = 4;   // points per thread per trip of the maxima reductions
 
 __device__ __forceinline__ float norm2_or_inf(v3 u)
 {
@@ -45,9 +46,22 @@ __global__ __launch_bounds__(kRedBlock) void k_norm_maxima(const float* __restri
     __shared__ float s_m[2][kRedBlock / 64];
     const int n = nv > na ? nv : na;
     float m0 = 0.0f, m1 = 0.0f;
-    for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
-        if (i < nv) m0 = fmax_(m0, norm2_or_inf(ld3(v, i)));
-        if (i < na) m1 = fmax_(m1, norm2_or_inf(ld3(a, i)));
+    // kRedUnroll points per trip, their loads issued together (a zero vector
+    // leaves a maximum unchanged; maxima do not depend on the order)
+    const int stride = gridDim.x * kRedBlock;
+    for (int i0 = blockIdx.x * kRedBlock + threadIdx.x; i0 < n; i0 += kRedUnroll * stride) {
+        v3 vv[kRedUnroll], aa[kRedUnroll];
+#pragma unroll
+        for (int u = 0; u < kRedUnroll; ++u) {
+            const int i = i0 + u * stride;
+            vv[u] = i < nv ? ld3(v, i) : V(0.0f, 0.0f, 0.0f);
+            aa[u] = i < na ? ld3(a, i) : V(0.0f, 0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int u = 0; u < kRedUnroll; ++u) {
+            m0 = fmax_(m0, norm2_or_inf(vv[u]));
+            m1 = fmax_(m1, norm2_or_inf(aa[u]));
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -95,11 +109,23 @@ __global__ __launch_bounds__(kRedBlock) void k_scene_maxima(const fs_scene* __re
 {
     __shared__ float s_m[2][kRedBlock / 64];
     const fs_scene sc = scenes[blockIdx.y];
+    if (blockIdx.x * kRedBlock >= sc.particle_count) return;   // past a smaller scene: nothing to add
     float m0 = 0.0f, m1 = 0.0f;
-    for (int k = blockIdx.x * kRedBlock + threadIdx.x; k < sc.particle_count; k += gridDim.x * kRedBlock) {
-        const int i = sc.particle_offset + k;
-        m0 = fmax_(m0, norm2_or_inf(ld3(v, i)));
-        if (use_accel) m1 = fmax_(m1, norm2_or_inf(ld3(a, i)));
+    const int stride = gridDim.x * kRedBlock;
+    for (int k0 = blockIdx.x * kRedBlock + threadIdx.x; k0 < sc.particle_count; k0 += kRedUnroll * stride) {
+        v3 vv[kRedUnroll], aa[kRedUnroll];
+#pragma unroll
+        for (int u = 0; u < kRedUnroll; ++u) {
+            const int k = k0 + u * stride;
+            const bool in = k < sc.particle_count;
+            vv[u] = in ? ld3(v, sc.particle_offset + k) : V(0.0f, 0.0f, 0.0f);
+            aa[u] = (in && use_accel) ? ld3(a, sc.particle_offset + k) : V(0.0f, 0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int u = 0; u < kRedUnroll; ++u) {
+            m0 = fmax_(m0, norm2_or_inf(vv[u]));
+            m1 = fmax_(m1, norm2_or_inf(aa[u]));
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -205,9 +231,10 @@ static int32_t launch_cfl(fs_scene* scenes, int32_t num_scenes, int32_t max_scen
                           double viscosity, double frame_dt, int32_t substeps, double* remaining, uint32_t* maxima,
                           int32_t* status, hipStream_t s)
 {
-    // blocks per scene: enough for the largest scene (about 8 particles per
-    // lane), at most 128
-    const int per_scene = max_scene_particles > 0 ? std::min(128, blocks_for(max_scene_particles, kRedBlock * 8)) : 8;
+    // blocks per scene: enough for the largest scene at about 32 particles
+    // per lane (8 trips of kRedUnroll), at most 128; a block past a smaller
+    // scene's particles returns at once
+    const int per_scene = max_scene_particles > 0 ? std::min(128, blocks_for(max_scene_particles, kRedBlock * 32)) : 8;
     FS_LAUNCH(k_scene_maxima, dim3(per_scene, num_scenes), dim3(kRedBlock), 0, s, scenes, v, a, use_accel, maxima);
     FS_LAUNCH(k_scene_dt, dim3(blocks_for(num_scenes, 64)), dim3(64), 0, s, scenes, num_scenes, maxima, stiffness,
               exponent, rest_density, viscosity, frame_dt, substeps, remaining, status);
