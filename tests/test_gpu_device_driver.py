@@ -68,3 +68,26 @@ def test_dfsph_frames_match_the_host_driver_bitwise():
 def test_unknown_method_is_refused():
     with pytest.raises(ValueError):
         DeviceResidentDamBreak(make_scene(512), "apic", device="cuda:0")
+
+
+@pytest.mark.parametrize("method", ["wcsph", "dfsph"])
+def test_lists_grow_between_device_frames(method):
+    """Both drivers start from 8-entry neighbour lists, which every particle
+    overflows.  The host driver's solver grows its lists after a step; the
+    device-resident driver grows them before a frame and captures its graph
+    again.  The frames stay bit-identical to the host driver's."""
+    scene = make_scene(3000)
+    host = DamBreakSimulation(scene, method, device="cuda:0")
+    dev = DeviceResidentDamBreak(scene, method, device="cuda:0")
+    for sim in (host, dev.sim):
+        sim.solver._alloc_work(8)
+        assert sim.solver.adaptive_lists
+    for _ in range(3):
+        host.step()
+        dev.step()
+    print(f"{method}: lists grown to {host.solver.neighbor_list_capacity} (host) and "
+          f"{dev.sim.solver.neighbor_list_capacity} (device)", flush=True)
+    assert host.solver.neighbor_list_capacity > 8 and dev.sim.solver.neighbor_list_capacity > 8
+    np.testing.assert_array_equal(dev.positions.cpu().numpy(), host.state_0.particle_q.numpy())
+    np.testing.assert_array_equal(dev.velocities.cpu().numpy(), host.state_0.particle_qd.numpy())
+    np.testing.assert_array_equal(dev.density.cpu().numpy(), host.state_0.sph.rho.numpy())
