@@ -10,12 +10,19 @@
 //
 //  1. stages the query's candidate slots, in visit order, 64 per chunk, in
 //     LDS (the 27-cell neighbourhood, once for the whole group);
-//  2. loads each chunk's candidate positions once (one candidate per lane)
-//     and tests them against every particle of the group, whose relative
-//     position and threshold are read from LDS by broadcast: per particle and
-//     chunk three FMAs, one compare and a two-word select;
+//  2. loads each chunk's candidate positions once (one candidate per lane,
+//     kCoopBatch chunks at a time) and tests them against every particle of
+//     the group, whose relative position and threshold are read with
+//     readlane: per particle and chunk three FMAs, one compare and a
+//     two-word select;
 //  3. keeps, per particle and chunk, the 64-bit mask of candidates that pass
 //     (bit order = visit order).
+//
+// Measured on gfx950 it is slower than the per-lane scan, at the developed
+// dam break (groups of about 6 particles: the per-group set-up costs as much
+// as the tests it shares) and on clustered points alike; the autotuner keeps
+// two variants of the WCSPH density pass and measures them on every input it
+// tunes.
 //
 // The test is the identity |p-c|^2 = |c'|^2 - 2 p'.c' + |p'|^2 on
 // coordinates relative to the group's first particle o (p' = p - o,
