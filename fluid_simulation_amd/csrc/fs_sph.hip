@@ -1957,7 +1957,8 @@ constexpr Variant kScanVariants[] = {{128, 48, 4, 1, 2}, {64, 24, 4, 1},     {64
                                      {128, 24, 4, 1},    {128, 32, 4, 1},    {128, 16, 4, 1},    {256, 32, 4, 1},
                                      {256, 24, 4, 1},    {256, 16, 4, 1},    {64, 40, 4, 1},     {64, 56, 4, 1},
                                      {64, 48, 4, 1},     {128, 32, 4, 1, 2}, {128, 64, 2, 1, 2}, {256, 48, 2, 1, 4},
-                                     {256, 32, 2, 1, 4},
+                                     {256, 32, 2, 1, 4}, {128, 40, 4, 1, 2}, {128, 56, 4, 1, 2},
+                                     {256, 40, 4, 1, 2},
                                      // cooperative cell scan (fs_coop.h): 64 lpp threads, cap chunk
                                      // slots per tile, coop chunks per group (the second holds the
                                      // 2,048-candidate queries of dense inputs).  Measured slower than
@@ -1970,7 +1971,8 @@ constexpr Variant kReplayVariants[] = {{128, 16, 4, 1, 2}, {128, 16, 4, 1},    {
 // DFSPH and PBF
 constexpr Variant kSolverScanVariants[] = {{128, 48, 4, 1, 2}, {64, 40, 4, 1},     {64, 56, 4, 1},
                                            {64, 32, 4, 1},     {128, 32, 4, 1},    {64, 48, 4, 1},
-                                           {128, 32, 4, 1, 2}, {256, 48, 2, 1, 4}, {256, 32, 2, 1, 4}};
+                                           {128, 32, 4, 1, 2}, {256, 48, 2, 1, 4}, {256, 32, 2, 1, 4},
+                                           {128, 40, 4, 1, 2}, {128, 56, 4, 1, 2}, {256, 40, 4, 1, 2}};
 constexpr Variant kSolverReplayVariants[] = {{256, 16, 4, 1, 4}, {64, 16, 4, 1},     {128, 16, 4, 1},
                                              {512, 16, 4, 1},    {256, 16, 8, 1},    {64, 16, 8, 1},
                                              {256, 16, 4, 1},    {256, 16, 2, 1, 4}, {256, 16, 4, 1, 2},
