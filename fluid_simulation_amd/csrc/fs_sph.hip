@@ -1967,12 +1967,18 @@ constexpr Variant kScanVariants[] = {{128, 48, 4, 1, 2}, {64, 24, 4, 1},     {64
 constexpr Variant kReplayVariants[] = {{128, 16, 4, 1, 2}, {128, 16, 4, 1},    {256, 16, 4, 1},    {512, 16, 4, 1},
                                        {64, 16, 8, 1},     {256, 16, 8, 1},    {64, 16, 4, 2},     {256, 16, 4, 2},
                                        {256, 16, 4, 1, 4}, {256, 16, 2, 1, 4}, {64, 16, 4, 1},     {256, 16, 4, 1, 2}};
-// the scanning (density/factor, density, lambda) and list-replaying passes of
-// DFSPH and PBF
+// the scanning (density, lambda) and list-replaying (non-pressure, delta,
+// XSPH) passes of PBF
 constexpr Variant kSolverScanVariants[] = {{128, 48, 4, 1, 2}, {64, 40, 4, 1},     {64, 56, 4, 1},
                                            {64, 32, 4, 1},     {128, 32, 4, 1},    {64, 48, 4, 1},
                                            {128, 32, 4, 1, 2}, {256, 48, 2, 1, 4}, {256, 32, 2, 1, 4},
                                            {128, 40, 4, 1, 2}, {128, 56, 4, 1, 2}, {256, 40, 4, 1, 2}};
+// DFSPH's scanning pass (density/factor): the same shapes, the deeper
+// survivor list first (its 35-neighbour lists at the developed dam break)
+constexpr Variant kDfsphScanVariants[] = {{128, 56, 4, 1, 2}, {64, 40, 4, 1},     {64, 56, 4, 1},
+                                          {64, 32, 4, 1},     {128, 32, 4, 1},    {64, 48, 4, 1},
+                                          {128, 32, 4, 1, 2}, {256, 48, 2, 1, 4}, {256, 32, 2, 1, 4},
+                                          {128, 40, 4, 1, 2}, {128, 48, 4, 1, 2}, {256, 40, 4, 1, 2}};
 constexpr Variant kSolverReplayVariants[] = {{256, 16, 4, 1, 4}, {64, 16, 4, 1},     {128, 16, 4, 1},
                                              {512, 16, 4, 1},    {256, 16, 8, 1},    {64, 16, 8, 1},
                                              {256, 16, 4, 1},    {256, 16, 2, 1, 4}, {256, 16, 4, 1, 2},
@@ -1993,10 +1999,17 @@ enum TunePass : int {
     kTunePbfScan = FS_TUNE_PBF_SCAN,
     kTunePbfReplay = FS_TUNE_PBF_REPLAY,
 };
-enum VariantTable : int { kTabScan = 0, kTabReplay = 1, kTabSolverScan = 2, kTabSolverReplay = 3, kTabDfsphReplay = 4 };
+enum VariantTable : int {
+    kTabScan = 0,
+    kTabReplay = 1,
+    kTabSolverScan = 2,
+    kTabSolverReplay = 3,
+    kTabDfsphReplay = 4,
+    kTabDfsphScan = 5
+};
 constexpr int kTunePasses = 8;   // FS_TUNE_WCSPH_DENSITY .. FS_TUNE_PBF_REPLAY
-constexpr int kTuneTable[kTunePasses] = {kTabScan,       kTabReplay,      kTabScan,       kTabReplay,
-                                         kTabSolverScan, kTabDfsphReplay, kTabSolverScan, kTabSolverReplay};
+constexpr int kTuneTable[kTunePasses] = {kTabScan,      kTabReplay,      kTabScan,       kTabReplay,
+                                         kTabDfsphScan, kTabDfsphReplay, kTabSolverScan, kTabSolverReplay};
 
 constexpr int table_size(int t)
 {
@@ -2004,7 +2017,8 @@ constexpr int table_size(int t)
            : t == kTabReplay ? (int)(sizeof(kReplayVariants) / sizeof(Variant))
            : t == kTabSolverScan ? (int)(sizeof(kSolverScanVariants) / sizeof(Variant))
            : t == kTabSolverReplay ? (int)(sizeof(kSolverReplayVariants) / sizeof(Variant))
-                                   : (int)(sizeof(kDfsphReplayVariants) / sizeof(Variant));
+           : t == kTabDfsphScan ? (int)(sizeof(kDfsphScanVariants) / sizeof(Variant))
+                                : (int)(sizeof(kDfsphReplayVariants) / sizeof(Variant));
 }
 constexpr Variant variant_at(int t, int i)
 {
@@ -2012,7 +2026,8 @@ constexpr Variant variant_at(int t, int i)
            : t == kTabReplay ? kReplayVariants[i]
            : t == kTabSolverScan ? kSolverScanVariants[i]
            : t == kTabSolverReplay ? kSolverReplayVariants[i]
-                                   : kDfsphReplayVariants[i];
+           : t == kTabDfsphScan ? kDfsphScanVariants[i]
+                                : kDfsphReplayVariants[i];
 }
 
 static std::atomic<int> g_tune[kTunePasses];
@@ -2028,6 +2043,7 @@ static Variant tuned_variant(int pass)
     case kTabReplay: return kReplayVariants[v];
     case kTabSolverScan: return kSolverScanVariants[v];
     case kTabSolverReplay: return kSolverReplayVariants[v];
+    case kTabDfsphScan: return kDfsphScanVariants[v];
     default: return kDfsphReplayVariants[v];
     }
 }
@@ -2641,7 +2657,7 @@ static int32_t df_density_factor(const DfPlan& p, bool after_integrate)
     dfa.sca = p.sca;
     dfa.vel = p.vel;
     dfa.dt_device = p.prm->dt_device;
-    return with_variant<kTabSolverScan>(tuned(kTuneDfsphScan), [&]<Variant V>() -> int32_t {
+    return with_variant<kTabDfsphScan>(tuned(kTuneDfsphScan), [&]<Variant V>() -> int32_t {
         FS_LAUNCH((k_dfsph_density_factor<V.block, V.cap, V.lpp, V.unroll>), dim3(blocks_for(p.n, V.block / V.lpp)),
                   dim3(V.block), 0, p.s, dfa);
         return FS_OK;
