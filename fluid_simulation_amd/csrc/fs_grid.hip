@@ -400,6 +400,13 @@ __global__ __launch_bounds__(kElemBlock) void k_scatter(int n, const int* __rest
     count[c] = 0;
 }
 
+constexpr int kFixBatch = 8;   // bucket members whose indices the fix-up loads at once
+
+// The gathered streams are template flags (POS: particle data at all;
+// PACKED: the build's packed records; VEL: velocities; SCENE: scene ids), so
+// every load of a point's data is straight-line code issued together with
+// the bucket-range loads, ahead of the rank loop.
+template <bool POS, bool PACKED, bool VEL, bool SCENE>
 __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __restrict__ scatter,
                                                            const int* __restrict__ start, int* __restrict__ point_ids,
                                                            int* __restrict__ point_cells, GatherOp ga)
@@ -409,14 +416,13 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __
     const int2 e = scatter[k];
     const int i = e.x;
     const int c = e.y;
-    // the point's gathered data depends only on i: its loads are issued
-    // before the rank loop, whose latency then covers theirs
+    const int sa = start[c], sb = start[c + 1];
     v3 x = V(0.0f, 0.0f, 0.0f), u = V(0.0f, 0.0f, 0.0f);
     float m = 0.0f, rd = 0.0f;
     int fl = 0, mk = 0, sc = 0;
-    if (ga.pos) {
+    if constexpr (POS) {
         x = ld3(ga.x, i);
-        if (ga.packed) {
+        if constexpr (PACKED) {
             const int4 pk = ga.packed[i];
             m = __int_as_float(pk.x);
             rd = __int_as_float(pk.y);
@@ -428,16 +434,25 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __
             fl = ga.flags[i];
             if (ga.mask) mk = ga.mask[i];
         }
-        if (ga.scene_of) sc = ga.scene_of[i];
-        if (ga.v) u = ld3(ga.v, i);
+        if constexpr (SCENE) sc = ga.scene_of[i];
+        if constexpr (VEL) u = ld3(ga.v, i);
     }
-    const int a = min(max(start[c], 0), n), b = min(max(start[c + 1], a), n);
+    // rank = members of the bucket with a smaller index; each trip loads
+    // kFixBatch members' indices at once (slots clamped into the bucket,
+    // those past its end not counted)
+    const int a = min(max(sa, 0), n), b = min(max(sb, a), n);
     int r = 0;
-    for (int j = a; j < b; ++j) r += (scatter[j].x < i) ? 1 : 0;
+    for (int j = a; j < b; j += kFixBatch) {
+        int t[kFixBatch];
+#pragma unroll
+        for (int q = 0; q < kFixBatch; ++q) t[q] = scatter[min(j + q, b - 1)].x;
+#pragma unroll
+        for (int q = 0; q < kFixBatch; ++q) r += (j + q < b && t[q] < i) ? 1 : 0;
+    }
     const int f = a + r;
     point_ids[f] = i;
     point_cells[f] = c;
-    if (ga.pos) {
+    if constexpr (POS) {
         ga.pos[f] = make_float4(x.x, x.y, x.z, m);
         // compute_particle_volume (solver_wcsph.py:109-116) times rest density,
         // as compute_density forms mass_nei (solver_wcsph.py:148)
@@ -446,8 +461,8 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __
         ga.sflags[f] = act | ((act && mk == 0) ? kDynamic : 0);
         if (ga.any_inactive && __ballot(!act) && (threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1)
             atomicOr(ga.any_inactive, 1);
-        if (ga.scene_of) ga.sscene[f] = sc;
-        if (ga.v) ga.vel[f] = make_float4(u.x, u.y, u.z, 0.0f);
+        if constexpr (SCENE) ga.sscene[f] = sc;
+        if constexpr (VEL) ga.vel[f] = make_float4(u.x, u.y, u.z, 0.0f);
     }
 }
 
@@ -549,8 +564,27 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
               reinterpret_cast<int2*>(g->scatter_ids),
               g->cell_count, g->scan_state, ntiles);
     if (nb == 0) return FS_OK;
-    FS_LAUNCH(k_fix_gather, dim3(nb), dim3(kElemBlock), 0, s, n, reinterpret_cast<const int2*>(g->scatter_ids),
-              g->cell_start, g->point_ids, g->point_cells, ga);
+    const int2* sids = reinterpret_cast<const int2*>(g->scatter_ids);
+    const bool pos = ga.pos != nullptr, packed = pos && ga.packed, vel = pos && ga.v, scene = pos && ga.scene_of;
+#define FIX_LAUNCH(P, K, V, S)                                                                                   \
+    FS_LAUNCH((k_fix_gather<P, K, V, S>), dim3(nb), dim3(kElemBlock), 0, s, n, sids, g->cell_start, g->point_ids, \
+              g->point_cells, ga)
+    if (!pos) FIX_LAUNCH(false, false, false, false);
+    else if (!packed) {
+        if (vel) FIX_LAUNCH(true, false, true, false);
+        else FIX_LAUNCH(true, false, false, false);
+        if (scene) {
+            set_error("grid build: scene ids need packed records");
+            return FS_ERR_ARG;
+        }
+    } else if (scene) {
+        if (vel) FIX_LAUNCH(true, true, true, true);
+        else FIX_LAUNCH(true, true, false, true);
+    } else {
+        if (vel) FIX_LAUNCH(true, true, true, false);
+        else FIX_LAUNCH(true, true, false, false);
+    }
+#undef FIX_LAUNCH
     return FS_OK;
 }
 
