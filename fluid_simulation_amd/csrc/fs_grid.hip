@@ -207,12 +207,17 @@ __device__ __forceinline__ int hash_point(const HashArgs& a, int i)
 }
 
 // The per-point words the fix-up gathers besides the positions and
-// velocities, packed into one 16-byte record of point i.
-__device__ __forceinline__ void pack_point(const GatherOp& ga, int i)
+// velocities, packed into one 16-byte record of point i: (mass, rest
+// density, flags, mask).  An absent array's word is loaded from the flags
+// array instead (always present, at least n words), so the four loads are
+// straight-line code; the fix-up reads such a word as 0.
+__device__ __forceinline__ int4 pack_load(const GatherOp& ga, int i)
 {
-    const float m = ga.mass ? ga.mass[i] : 0.0f;
-    const float rd = ga.wvol ? ga.rest_density[i] : 0.0f;
-    ga.packed[i] = make_int4(__float_as_int(m), __float_as_int(rd), ga.flags[i], ga.mask ? ga.mask[i] : 0);
+    const int* fl = ga.flags;
+    const int m = (ga.mass ? reinterpret_cast<const int*>(ga.mass) : fl)[i];
+    const int rd = (ga.wvol ? reinterpret_cast<const int*>(ga.rest_density) : fl)[i];
+    const int mk = (ga.mask ? ga.mask : fl)[i];
+    return make_int4(m, rd, fl[i], mk);
 }
 
 // Bucket every point and take a slot rank from its bucket counter.  Lanes are
@@ -223,16 +228,28 @@ __device__ __forceinline__ void pack_point(const GatherOp& ga, int i)
 // it.  A bucket split over several runs just takes several atomics.  Ranks only
 // need to be distinct: k_fix_gather orders each bucket by original index
 // afterwards.
-template <int MODE>
+// ORDERED: a.order is set; PACK: a.pack.packed is set.  Both as template
+// flags, the processing-order load and the packed record's loads are
+// straight-line code issued together (lanes past n load point n - 1 and
+// write nothing).
+template <int MODE, bool ORDERED, bool PACK>
 __global__ __launch_bounds__(kElemBlock) void k_hash_count(HashArgs a)
 {
     const int t = blockIdx.x * kElemBlock + threadIdx.x;
     if (a.pre.control_reset && t < 16) a.pre.control_reset[t] = 0;
     const bool valid = t < a.n;   // the valid lanes of a wave are a prefix
-    const int i = valid ? (a.order ? a.order[t] : t) : 0;
+    const int tl = min(t, a.n - 1);
+    int i = 0;
+    int4 pk = make_int4(0, 0, 0, 0);
+    if (a.n > 0) {
+        i = ORDERED ? a.order[tl] : tl;
+        // the packed record of point t (original order: coalesced), stored
+        // after the bucket so that its loads overlap the bucket's
+        if constexpr (PACK) pk = pack_load(a.pack, tl);
+    }
     int c = -1;
     if (valid) c = hash_point<MODE>(a, i);
-    if (valid && a.pack.packed) pack_point(a.pack, t);   // original order: coalesced
+    if (PACK && valid) a.pack.packed[t] = pk;
     const int lane = threadIdx.x & 63;
     const int prev = __shfl_up(c, 1, 64);
     const bool head = valid && (lane == 0 || prev != c);
@@ -423,11 +440,12 @@ __global__ __launch_bounds__(kElemBlock) void k_fix_gather(int n, const int2* __
     if constexpr (POS) {
         x = ld3(ga.x, i);
         if constexpr (PACKED) {
+            // words of absent arrays (pack_load) read as 0
             const int4 pk = ga.packed[i];
-            m = __int_as_float(pk.x);
-            rd = __int_as_float(pk.y);
+            m = ga.mass ? __int_as_float(pk.x) : 0.0f;
+            rd = ga.wvol ? __int_as_float(pk.y) : 0.0f;
             fl = pk.z;
-            mk = pk.w;
+            mk = ga.mask ? pk.w : 0;
         } else {
             if (ga.mass) m = ga.mass[i];
             if (ga.wvol) rd = ga.rest_density[i];
@@ -550,10 +568,19 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
     }
     if (nb > 0 || pre.control_reset) {
         const int nbh = nb > 0 ? nb : 1;
+        const bool pack = ha.pack.packed != nullptr;
         switch (ha.pre.mode) {
-        case kPreIntegrate: FS_LAUNCH(k_hash_count<kPreIntegrate>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
-        case kPrePredict: FS_LAUNCH(k_hash_count<kPrePredict>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
-        default: FS_LAUNCH(k_hash_count<kPrePlain>, dim3(nbh), dim3(kElemBlock), 0, s, ha); break;
+#define HASH_LAUNCH(M)                                                                                       \
+    do {                                                                                                     \
+        if (ha.order && pack) FS_LAUNCH((k_hash_count<M, true, true>), dim3(nbh), dim3(kElemBlock), 0, s, ha); \
+        else if (ha.order) FS_LAUNCH((k_hash_count<M, true, false>), dim3(nbh), dim3(kElemBlock), 0, s, ha);  \
+        else if (pack) FS_LAUNCH((k_hash_count<M, false, true>), dim3(nbh), dim3(kElemBlock), 0, s, ha);      \
+        else FS_LAUNCH((k_hash_count<M, false, false>), dim3(nbh), dim3(kElemBlock), 0, s, ha);              \
+    } while (0)
+        case kPreIntegrate: HASH_LAUNCH(kPreIntegrate); break;
+        case kPrePredict: HASH_LAUNCH(kPrePredict); break;
+        default: HASH_LAUNCH(kPrePlain); break;
+#undef HASH_LAUNCH
         }
     }
     const int ntiles = scan_tiles(g);
