@@ -97,7 +97,7 @@ static void prof_collect()
 }
 
 constexpr int kScanBlock = 256;
-constexpr int kScanItems = 16;
+constexpr int kScanItems = 32;
 constexpr int kScanTile = kScanBlock * kScanItems;
 
 static inline size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -286,8 +286,8 @@ __device__ __forceinline__ uint64_t scan_desc(uint32_t status, int value)
 
 // Single-pass exclusive scan with decoupled look-back.  Tile = blockIdx.x:
 // workgroups are dispatched in index order, so every predecessor of a
-// resident tile has started and will publish.  Each wave scans 1024 counters
-// of the tile in 4 rounds of 64 lanes x int4 (every load and store
+// resident tile has started and will publish.  Each wave scans 2048 counters
+// of the tile in 8 rounds of 64 lanes x int4 (every load and store
 // instruction covers 1 KB of contiguous memory), carrying the running total
 // from round to round.
 __global__ __launch_bounds__(kScanBlock) void k_bucket_scan(const int* __restrict__ in, int* __restrict__ out,
