@@ -2030,6 +2030,22 @@ constexpr Variant variant_at(int t, int i)
                                 : kDfsphReplayVariants[i];
 }
 
+// Every launch shape covers at least kMinPassBlock particles per workgroup:
+// fs_work_bytes sizes the rate pass's per-workgroup residual partials (and
+// the tile mappings assume whole 64-slot tiles) on that bound.
+constexpr bool table_fits(int t)
+{
+    for (int i = 0; i < table_size(t); ++i) {
+        const Variant v = variant_at(t, i);
+        if (v.lpp < 1 || v.block % v.lpp != 0 || v.block / v.lpp < kMinPassBlock || (v.block / v.lpp) % 64 != 0)
+            return false;
+    }
+    return true;
+}
+static_assert(table_fits(kTabScan) && table_fits(kTabReplay) && table_fits(kTabSolverScan) &&
+                  table_fits(kTabSolverReplay) && table_fits(kTabDfsphScan) && table_fits(kTabDfsphReplay),
+              "a launch variant covers fewer than kMinPassBlock particles per workgroup");
+
 static std::atomic<int> g_tune[kTunePasses];
 static std::mutex g_tune_mutex;
 
