@@ -491,7 +491,7 @@ __device__ __forceinline__ void wcsph_density_core(const WcsphDensityArgs& a, co
     auto term = [&](int k, const NbrXS& nb) -> float {
         if (!(nb.flags & kActive)) return 0.0f;
         v3 dist = vsub(xyz(xi), xyz(nb.x));
-        return nb.s * cubic_w(vlen(dist), sm);
+        return nb.s * cubic_w(vlen_nr(dist), sm);
     };
     auto fold = [&](float t) { rho_temp += t; };
     loop(xi, load, term, fold);
@@ -933,7 +933,7 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_force(WadjForceArgs a)
             if (j == s || !(nb.flags & kActive)) return t;
             const float4 xj4 = nb.x;
             const v3 r = vsub(xk, xyz(xj4));
-            const float d = vlen(r);
+            const float d = vlen_nr(r);
             if (!(d / h <= 1.0f)) return t;
             const v3 G = cubic_grad_r(r, d, h);
             const float4 aj4 = nb.abar;
@@ -1030,7 +1030,7 @@ __global__ __launch_bounds__(kPassBlock) void k_wadj_density(WadjDensityArgs a)
             [&](int j, const NbrXV& nb) -> v3 {
                 if (j == s || !(nb.flags & kActive)) return V(0.0f, 0.0f, 0.0f);
                 const v3 r = vsub(xk, xyz(nb.x));
-                const float d = vlen(r);
+                const float d = vlen_nr(r);
                 if (!(d / h <= 1.0f)) return V(0.0f, 0.0f, 0.0f);
                 return smul(rb_k * nb.v.x + nb.v.y * wv_k, cubic_grad_r(r, d, h));
             },
@@ -1558,7 +1558,7 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_density(PbfDensityArgs a)
             [&](int k, const NbrXS& nb) -> float {
                 if (!(nb.flags & kActive)) return 0.0f;
                 v3 r = vsub(xi, xyz(nb.x));
-                return nb.s * poly6_w(vlen(r), h);
+                return nb.s * poly6_w(vlen_nr(r), h);
             },
             [&](float t) { rho_temp += t; });
     } else if (c.L.mode == kListBuild && lead) {
@@ -1621,7 +1621,7 @@ __device__ __forceinline__ v3 pbf_nonpressure_sum(const PbfNonPressureArgs& a, i
             if (k == s || !(nb.flags & kActive)) return t;
             const float4 xj = nb.x;
             const v3 d = vsub(xi, xyz(xj));
-            const float dist = vlen(d);
+            const float dist = vlen_nr(d);
             if constexpr (COH) t.coh = muls(smul(gamma / m_i * xj.w, d), poly6_w(fmax_(dist, pr), h));
             if (mu != 0.0f) {
                 const float4 vj = nb.v;
@@ -1730,7 +1730,7 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_lambda(PbfLambdaArgs a)
             if (!(nb.flags & kActive)) return t;
             const float4 xj = nb.x;
             v3 xij = vsub(xi, xyz(xj));
-            float r = vlen(xij);
+            float r = vlen_nr(xij);
             t.count = (k != s && r < h) ? 1 : 0;
             float mass_j = xj.w;
             t.rho = mass_j * poly6_w(r, h);
@@ -1800,7 +1800,7 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_delta(PbfDeltaArgs a)
                     if (k == s || !(nb.flags & kActive)) return V(0.0f, 0.0f, 0.0f);
                     const float4 xj = nb.x;
                     v3 xij = vsub(xi, xyz(xj));
-                    const float rn = vlen(xij);
+                    const float rn = vlen_nr(xij);
                     v3 grad = spiky_grad_r(xij, rn, h);
                     float scorr = 0.0f;
                     if (SCORR) scorr = compute_scorr(rn, h, a.scorr_k, a.scorr_n, a.scorr_w_delta_q);
@@ -1861,7 +1861,7 @@ __global__ __launch_bounds__(BLOCK) void k_pbf_xsph(PbfXsphArgs a)
             if (k == s || !(nb.flags & kActive)) return V(0.0f, 0.0f, 0.0f);
             const float4 xj = nb.x;
             const float4 vj = nb.v;
-            float w = poly6_w(vlen(vsub(x, xyz(xj))), h);
+            float w = poly6_w(vlen_nr(vsub(x, xyz(xj))), h);
             return muls(smul(vj.w, vsub(xyz(vj), vi)), w);
         },
         [&](const v3& t) { corr = vadd(corr, t); });
@@ -1912,7 +1912,7 @@ __global__ __launch_bounds__(BLOCK) void k_separation(SepArgs a)
             [&](int k, const NbrX& nb) {
                 if (k == s || !(nb.flags & kActive)) return;
                 const v3 rij = vsub(xi, xyz(nb.x));
-                const float d = vlen(rij);
+                const float d = vlen_nr(rij);
                 if (d > 1.0e-8f && d < rest) {
                     disp = vadd(disp, muls(vdivs(rij, d), strength * (rest - d)));
                     count += 1.0f;
