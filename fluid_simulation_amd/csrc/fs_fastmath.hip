@@ -1,6 +1,6 @@
 // fs_fastmath.hip -- on-device comparison of the shared-reciprocal quotient
-// and square-root forms of fs_common.h (div_nr, sqrt_nr, cubic_len_grad,
-// cubic_w on a Smooth) with the correctly rounded forms hipcc emits for '/'
+// and square-root forms of fs_common.h (div_nr, sqrt_nr, vlen_nr,
+// cubic_len_grad, cubic_w on a Smooth) with the correctly rounded forms hipcc emits for '/'
 // and sqrtf.  A diagnostic: the neighbour passes use the fast forms only on
 // the operand ranges this kernel covers.
 #include "fs_internal.h"
@@ -42,7 +42,8 @@ __device__ __forceinline__ float component(uint64_t u, float h)
 // mode 2: x = bits(first + t), d = bits(seed): div_nr vs '/'
 // mode 3: a random separation r and smoothing length h (mostly in
 //         [2^-20, 2^20), some outside): cubic_len_grad(r, smooth_of(h)) vs
-//         (vlen(r), cubic_grad(r, h)) and cubic_w on a Smooth vs on h
+//         (vlen(r), cubic_grad(r, h)), cubic_w on a Smooth vs on h, and
+//         vlen_nr(r) vs vlen(r)
 __global__ __launch_bounds__(kElemBlock) void k_check_fast_math(int mode, uint64_t first, uint64_t count,
                                                                 uint32_t seed, unsigned long long* out)
 {
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(kElemBlock) void k_check_fast_math(int mode, uint64
             const float len = vlen(r);
             const v3 g = cubic_grad_r(r, len, h);
             bad = differ(lg.len, len) || differ(lg.grad.x, g.x) || differ(lg.grad.y, g.y) || differ(lg.grad.z, g.z) ||
-                  differ(cubic_w(len, sm), cubic_w(len, h));
+                  differ(cubic_w(len, sm), cubic_w(len, h)) || differ(vlen_nr(r), len);
         }
         if (bad) {
             atomicAdd(&out[0], 1ull);
