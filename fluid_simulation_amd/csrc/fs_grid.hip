@@ -548,6 +548,10 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
         set_error("grid build: radius must be positive");
         return FS_ERR_ARG;
     }
+    if (ga.pos && ga.scene_of && !ga.packed) {
+        set_error("grid build: scene ids need packed records");
+        return FS_ERR_ARG;
+    }
     const bool reorder = g->num_points == n && g->cell_width > 0.0f;
     g->num_points = n;
     g->cell_width = radius;
@@ -600,10 +604,6 @@ int32_t build_grid(fs_grid* g, int n, float radius, const PreOp& pre, const Gath
     else if (!packed) {
         if (vel) FIX_LAUNCH(true, false, true, false);
         else FIX_LAUNCH(true, false, false, false);
-        if (scene) {
-            set_error("grid build: scene ids need packed records");
-            return FS_ERR_ARG;
-        }
     } else if (scene) {
         if (vel) FIX_LAUNCH(true, true, true, true);
         else FIX_LAUNCH(true, true, false, true);
