@@ -121,6 +121,32 @@ def test_compressed_block_one_substep(method, oracle):
 
 
 @pytest.mark.parametrize("method", ["wcsph", "dfsph", "pbf"])
+def test_coincident_and_nearly_coincident_particles(method, oracle):
+    """Degenerate separations: exact duplicates of block particles (r = 0,
+    where the kernel gradients return zero: r <= 1e-5 for the cubic spline,
+    r <= 1e-6 for the spiky gradient), and a pair 1e-15 apart near the
+    origin, whose squared distance (1e-30) is below the 2^-96 where the fast
+    square root hands over to sqrtf.  Against the oracle, as the other
+    one-substep cases."""
+    model0, cfg = _compressed_case(method, seed=4, inactive=False)
+    d = 0.02
+    base = model0.particle_q.numpy()
+    rng = np.random.default_rng(4)
+    dup = base[rng.choice(base.shape[0], 6, replace=False)]
+    tiny = np.array([[1.0e-15, 1.0e-15, 1.0e-15], [2.0e-15, 1.0e-15, 1.0e-15]], np.float32)
+    pts = np.vstack([base, dup, tiny]).astype(np.float32)
+    n = pts.shape[0]
+    vel = (rng.normal(size=(n, 3)) * 0.5).astype(np.float32)
+    mat = {"rest_density": 1000.0, "surface_tension": 0.01, "viscosity": 1.0e-3}
+    if method == "wcsph":
+        mat.update(stiffness=250000.0, exponent=7.0)
+    model = build_model(method, pts, vel, mass=1000.0 * d ** 3, radius=0.5 * d, mat=mat)
+    dt = 2.0e-4 if method == "wcsph" else 1.0e-3
+    solver, s_out, oout, extras = _run_pair(method, model, cfg, dt, oracle)
+    _check_outputs(method, solver, s_out, oout, extras)
+
+
+@pytest.mark.parametrize("method", ["wcsph", "dfsph", "pbf"])
 def test_cohesion_one_substep(method, oracle):
     """Surface tension 0.01, the reference's default material value
     (solver_wcsph.py:348): the cohesion term ((gamma/m_i*m_j)*d)*W of
