@@ -147,6 +147,29 @@ def test_coincident_and_nearly_coincident_particles(method, oracle):
 
 
 @pytest.mark.parametrize("method", ["wcsph", "dfsph", "pbf"])
+def test_block_far_from_the_origin(method, oracle):
+    """The compressed block moved to x = +1000 m, y = -700 m (bounds widened
+    to hold it): coordinates whose float32 spacing is ~6e-5 m, so every
+    separation carries rounding from the positions, and cell coordinates in
+    the tens of thousands, which the 128-bucket hash wraps."""
+    model0, cfg = _compressed_case(method, seed=5)
+    d = 0.02
+    pts = model0.particle_q.numpy() + np.array([1000.25, -700.5, 0.0], np.float32)
+    vel = model0.particle_qd.numpy()
+    flags = model0.particle_flags.numpy()
+    mask = model0.sph.particle_mask.numpy()
+    mat = {"rest_density": 1000.0, "surface_tension": 0.01, "viscosity": 1.0e-3}
+    if method == "wcsph":
+        mat.update(stiffness=250000.0, exponent=7.0)
+    model = build_model(method, pts.astype(np.float32), vel, mass=1000.0 * d ** 3, radius=0.5 * d, mat=mat,
+                        flags=flags, mask=mask)
+    cfg = replace(cfg, bound_width=2000.0, bound_height=2000.0)
+    dt = 2.0e-4 if method == "wcsph" else 1.0e-3
+    solver, s_out, oout, extras = _run_pair(method, model, cfg, dt, oracle)
+    _check_outputs(method, solver, s_out, oout, extras)
+
+
+@pytest.mark.parametrize("method", ["wcsph", "dfsph", "pbf"])
 def test_cohesion_one_substep(method, oracle):
     """Surface tension 0.01, the reference's default material value
     (solver_wcsph.py:348): the cohesion term ((gamma/m_i*m_j)*d)*W of
