@@ -102,6 +102,25 @@ def pmc_traffic(key: str):
     return None
 
 
+PMC_CACHE = os.path.join(ROOT, "profiles", "wcsph_dam_break_401856_pmc_cache.md")
+
+
+def pmc_cache(key: str):
+    """(L1 hit, L2 hit) fractions of one launch variant (variant_key), from
+    the committed rocprofv3 cache summary (TCP_TCC_READ_REQ against
+    TCP_TOTAL_CACHE_ACCESSES, TCC_HIT against TCC_MISS); None when the
+    variant was not measured."""
+    try:
+        with open(PMC_CACHE) as f:
+            for line in f:
+                if key in line and line.startswith("|"):
+                    cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                    return float(cells[4].rstrip("%")) / 100.0, float(cells[7].rstrip("%")) / 100.0
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
+
+
 PMC_SQ = os.path.join(ROOT, "profiles", "wcsph_dam_break_401856_pmc_sq.md")
 # wave64 FP32 VALU issue of the whole chip: one instruction per CU per cycle
 # (4 SIMDs of 16 lanes, 4 cycles each), 256 CUs at 2.4 GHz
@@ -888,6 +907,12 @@ def main():
                 "bytes_per_launch": PASS_BYTES[dom] * scene.count,
                 "traffic_source": "profiles/wcsph_dam_break_401856_pmc_traffic.md (HBM bytes per launch of the "
                                   "selected launch variant: FETCH_SIZE x2.0 + WRITE_SIZE x1.0)"}
+    if tkey in tuned:
+        hits = pmc_cache(variant_key(dom, tuned[tkey]))
+        if hits:
+            roofline.update(l1_hit=hits[0], l2_hit=hits[1],
+                            cache_source="profiles/wcsph_dam_break_401856_pmc_cache.md (the selected launch "
+                                         "variant's vector L1 and L2 hit rates)")
     valu = None
     if tkey in tuned:
         instr = pmc_valu_instructions(variant_key(dom, tuned[tkey]))

@@ -216,6 +216,26 @@ def test_struct_layouts_agree_with_header(tmp_path):
             assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
 
 
+def test_bench_reads_the_committed_profiles_for_the_default_variants():
+    """bench.py prices the dominant kernel's launch from the committed
+    rocprofv3 summaries: HBM traffic, VALU instructions and cache hit rates
+    of the default launch variants must be there."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from fluid_simulation_amd import autotune as at
+
+    for pass_name, kernel in (("wcsph_density", "k_wcsph_density"), ("wcsph_force", "k_wcsph_force")):
+        v0 = at.describe(pass_name)[0]
+        key = bench.variant_key(kernel, {"block": v0["block"], "list_depth": v0["list_depth"], "unroll": v0["unroll"],
+                                         "particles_per_thread": v0["particles_per_thread"],
+                                         "lanes_per_particle": v0["lanes_per_particle"], "coop": v0["coop"]})
+        assert bench.pmc_traffic(key) > 0, key
+        assert bench.pmc_valu_instructions(key) > 0, key
+        l1, l2 = bench.pmc_cache(key)
+        assert 0.0 < l1 <= 1.0 and 0.0 < l2 <= 1.0, key
+    assert bench.pmc_cache("k_no_such_kernel<1>") is None
+
+
 def test_bench_refuses_multi_gpu():
     res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                          text=True, timeout=300)
